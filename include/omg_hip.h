@@ -1,0 +1,291 @@
+/*
+ * omg_hip.h -- C ABI of the MI355X stochastic energy path tracer.
+ *
+ * A drop-in for omg-audio's tracer seams: a Rust `omg-hip` crate binds
+ * these symbols with `extern "C"` and implements the reference's traits
+ * over them (paths below are relative to the reference checkout).  Plain
+ * pointers and sizes only; every call returns an int status (OMG_OK = 0,
+ * negative = refused or failed: the caller keeps its CPU path, as the
+ * reference's backends do by returning `false` / `None`).
+ *
+ * Conventions kept from the reference seams:
+ *   - out-buffers are caller-owned and overwritten completely;
+ *   - the static mesh is uploaded once (omg_scene_create), overlays,
+ *     sources and listeners travel per call;
+ *   - caps are refused, never clamped or silently truncated;
+ *   - an omg_ctx is used by one thread at a time (the reference calls its
+ *     backends from one sim thread under a Mutex, late.rs:51, :93);
+ *     separate contexts may be used from separate threads.
+ *
+ * Echogram layout, per (source, listener) pair: bins [nbins][nbands] f32
+ * then dirs [nbins][3] f32, bin-major, exactly the reference Echogram
+ * (crates/omg-core/src/tracer.rs:18-25) with nbins = max_time / bin_dt.
+ */
+#ifndef OMG_HIP_H
+#define OMG_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMG_LAYOUT_VERSION 1u   /* bump on any struct change (cf. layout.rs:12, :91) */
+#define OMG_MAX_BANDS 8
+#define OMG_MAX_OVERLAYS 64     /* panel overlay cap (layout.rs:94): refused above */
+#define OMG_MAX_LISTENERS 64
+#define OMG_DOME_BINS 9         /* dome.rs:30 */
+
+#define OMG_OK 0
+#define OMG_ERR_NO_DEVICE (-1)  /* no usable GPU (omg-gpu/src/lib.rs:33-47 returns None) */
+#define OMG_ERR_INVALID (-2)    /* bad argument */
+#define OMG_ERR_CAP (-3)        /* a cap was exceeded: refused (omg-gpu/src/lib.rs:519-528) */
+#define OMG_ERR_VERSION (-4)    /* layout version mismatch (web/gpu.js:17-22) */
+#define OMG_ERR_DEVICE (-5)     /* HIP runtime failure */
+#define OMG_ERR_BUDGET (-6)     /* serving: HBM budget refusal */
+#define OMG_ERR_NOT_READY (-7)  /* async: no fresh result yet (late.rs:46-48) */
+#define OMG_ERR_OVERFLOW (-8)   /* a bounded device list overflowed; results not valid */
+
+typedef struct omg_ctx omg_ctx;
+typedef struct omg_scene omg_scene;
+
+/* material.rs:7-17, generalised to nbands <= OMG_MAX_BANDS */
+typedef struct {
+    float absorption[OMG_MAX_BANDS];
+    float scattering;
+    float transmission[OMG_MAX_BANDS];
+} omg_material;
+
+/* An axis-aligned overlay box with inline acoustics: door leaves, panes,
+ * furniture (scene.rs:84-94, WithPanels; GpuPanel in layout.rs:124-137). */
+typedef struct {
+    float min[3];
+    float max[3];
+    omg_material mat;
+} omg_overlay;
+
+/* Per-call tracer job (tracer.rs:12-16 constants become parameters). */
+typedef struct {
+    uint32_t n_rays;          /* rays per source */
+    uint32_t nbands;          /* 1..OMG_MAX_BANDS (the reference has 3) */
+    uint32_t max_bounces;     /* 64 in tracer.rs:103 */
+    uint32_t dir_band;        /* band weighting the arrival direction (1 in tracer.rs:123) */
+    uint64_t seed;            /* Philox key of the per-ray counter stream */
+    float bin_dt;             /* BIN_DT (tracer.rs:12) */
+    float max_time;           /* MAX_TIME (tracer.rs:13) */
+    float floor_rel;          /* relative energy floor, 1e-7 (tracer.rs:163, :179) */
+    float radius;             /* RECEIVER_RADIUS 0.5 (tracer.rs:15) */
+    float energy[OMG_MAX_BANDS];  /* source energy per band */
+} omg_trace_params;
+
+/* Per-ray parity record (optional output). */
+typedef struct {
+    uint32_t iters;           /* bounce-loop iterations = bounce-rays */
+    uint32_t term;            /* 0 escape, 1 time, 2 dead after transmit, 3 dead after reflect, 4 max bounces */
+    uint32_t hit_hash;        /* FNV-1a over the hit triangle ids */
+    uint32_t flags;           /* bit0 near-edge hit, bit1 exact-t tie */
+} omg_ray_record;
+
+/* Traversal counters (optional output): AABB tests, triangle tests,
+ * bounce-rays -- the inputs of the algorithmic-bytes roofline. */
+typedef struct {
+    uint64_t aabb_tests;
+    uint64_t tri_tests;
+    uint64_t bounce_rays;
+} omg_counters;
+
+typedef struct {
+    uint32_t n_tris, n_prims, n_nodes, max_depth;
+    uint64_t device_bytes;
+} omg_scene_info;
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* GpuTracer::new / GpuMeshTracer::new (omg-gpu/src/lib.rs:33-47): returns
+ * OMG_ERR_NO_DEVICE when no GPU is usable, OMG_ERR_VERSION when the caller
+ * was built against another layout. */
+int omg_init(int device, uint32_t layout_version, omg_ctx **out);
+void omg_shutdown(omg_ctx *ctx);
+/* Thread-local message for the last failing call on this thread. */
+const char *omg_last_error(void);
+int omg_device_name(omg_ctx *ctx, char *buf, uint32_t len);
+
+/* ------------------------------------------------------------ scenes */
+
+/* Mesh::with_surfaces_edges + build_bvh (mesh.rs:198-405) and the BVH
+ * upload of GpuMeshTracer::new (omg-gpu/src/lib.rs:274-287): positions
+ * [nv][3], indices [nt][3], per-triangle material / surface id /
+ * tessellation limit (tri_surface and tri_max_edge may be NULL: 0 and
+ * 4 m), materials [nm].  Tessellation and SAH reproduce the reference's
+ * primitives bit for bit. */
+int omg_scene_create(omg_ctx *ctx, const float *positions, uint32_t nv, const uint32_t *indices,
+                     uint32_t nt, const uint16_t *tri_material, const uint16_t *tri_surface,
+                     const float *tri_max_edge, const omg_material *materials, uint32_t nm,
+                     uint32_t nbands, omg_scene **out);
+/* MeshBuilder::tri + build (mesh.rs:108-165): a triangle soup [nt][3][3]
+ * with the reference's 0.1 mm vertex dedup applied first. */
+int omg_scene_create_soup(omg_ctx *ctx, const float *tri_verts, uint32_t nt,
+                          const uint16_t *tri_material, const uint16_t *tri_surface,
+                          const float *tri_max_edge, const omg_material *materials, uint32_t nm,
+                          uint32_t nbands, omg_scene **out);
+int omg_scene_info_get(omg_scene *scene, omg_scene_info *info);
+/* Host-only inspection of the scene build (no device needed): the
+ * MeshBuilder dedup (mesh.rs:108-114) and the packed primitives / paired
+ * nodes exactly as they would be uploaded.  prims_out rows are 12 words
+ * (a[3], tri, e1[3], mat, e2[3], surf); nodes_out rows are 16 words
+ * (lmin[3], lref, lmax[3], rref, rmin[3], 0, rmax[3], 0); either may be
+ * NULL to query the counts.  A leaf reference has bit 31 set, its count in
+ * bits 28..30 and its first primitive in bits 0..27. */
+int omg_dedup_soup(const float *tri_verts, uint32_t nt, float *positions_out,
+                   uint32_t *indices_out, uint32_t *nv_out);
+int omg_bvh_build_host(const float *positions, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                       const uint16_t *tri_material, const uint16_t *tri_surface,
+                       const float *tri_max_edge, float *prims_out, uint32_t prim_cap,
+                       uint32_t *n_prims, float *nodes_out, uint32_t node_cap, uint32_t *n_nodes,
+                       uint32_t *root_ref, uint32_t *max_depth);
+
+void omg_scene_destroy(omg_scene *scene);
+
+/* ------------------------------------------------------------ late-field tracer */
+
+/* WorldLateBackend::trace_world (crates/omg-scene/src/late.rs:81-91;
+ * omg-gpu/src/lib.rs:845-873), multi-source and multi-listener: every
+ * source traces params->n_rays rays, each scored against every listener
+ * (a ray's path does not depend on the listener).  Host buffers:
+ * sources [n_src][3], listeners [n_lis][3], overlays [n_ovl] (<= 64),
+ * bins [n_src][n_lis][nbins][nbands], dirs [n_src][n_lis][nbins][3].
+ * records [n_src][n_rays] and counters are optional (NULL). */
+int omg_trace_world(omg_scene *scene, const omg_trace_params *params, const float *sources,
+                    uint32_t n_src, const float *listeners, uint32_t n_lis,
+                    const omg_overlay *overlays, uint32_t n_ovl, float *bins, float *dirs,
+                    omg_ray_record *records, omg_counters *counters);
+
+/* LateBackend::trace (late.rs:28-40; omg-gpu/src/lib.rs:1051-1067) over the
+ * analytic Shoebox (scene.rs:25-82): size[3], walls[6] in the order
+ * x-, x+, y-, y+, z-, z+.  Same buffers as omg_trace_world. */
+int omg_trace_box(omg_ctx *ctx, const float size[3], const omg_material walls[6],
+                  uint32_t nbands, const omg_trace_params *params, const float *sources,
+                  uint32_t n_src, const float *listeners, uint32_t n_lis, float *bins,
+                  float *dirs, omg_ray_record *records, omg_counters *counters);
+
+/* Device-resident variant of omg_trace_world: sources, listeners and the
+ * outputs are device pointers, the launch is enqueued on `stream`
+ * (a hipStream_t, NULL = the context's stream) and the call returns
+ * without waiting.  Overlays are host memory, copied before the launch. */
+int omg_trace_world_async(omg_scene *scene, const omg_trace_params *params,
+                          const float *d_sources, uint32_t n_src, const float *d_listeners,
+                          uint32_t n_lis, const omg_overlay *overlays, uint32_t n_ovl,
+                          float *d_bins, float *d_dirs, omg_ray_record *d_records,
+                          void *stream);
+
+/* ------------------------------------------------------------ echogram post-processing */
+
+/* estimate_reverb (tracer.rs:205-257): rt60[nbands], level[nbands]. */
+int omg_estimate_reverb(const float *bins, uint32_t nbins, uint32_t nbands, float bin_dt,
+                        float *rt60, float *level);
+/* Echogram::late_direction (tracer.rs:62-77). */
+int omg_late_direction(const float *bins, const float *dirs, uint32_t nbins, uint32_t nbands,
+                       float bin_dt, float from_s, float dir_out[3], float *aniso);
+
+/* ------------------------------------------------------------ ambient dome */
+
+/* Dome panel (dome.rs:41-53): a rectangle in a vertical plane. axis 0 =
+ * plane of constant x, 1 = constant y.  glass != 0 transmits bounce rays
+ * with trans^2; a leaf (glass == 0) blocks them.  The straight-through ray
+ * pays trans^2 for either kind (dome.rs:407-420). */
+typedef struct {
+    uint32_t axis;
+    float plane;
+    float lat0, lat1, z0, z1;
+    uint32_t glass;
+    float trans[OMG_MAX_BANDS];
+} omg_dome_panel;
+
+typedef struct {
+    uint32_t n_fan;           /* N_RAYS, the golden-spiral fan size (dome.rs:34) */
+    uint32_t n_rays;          /* rays traced, <= n_fan, even stride (dome.rs:432-434) */
+    uint32_t max_events;      /* MAX_EVENTS 6 (dome.rs:35) */
+    uint32_t nbands;
+    float horizon;            /* HORIZON 80 m (dome.rs:36); the sky-sphere radius */
+    float through_len;        /* straight-through segment, 50 m (dome.rs:392) */
+} omg_dome_params;
+
+typedef struct {
+    uint32_t escaped;         /* the bounce path reached the sky */
+    uint32_t events;          /* events used by the bounce path */
+    uint32_t used_through;    /* the straight-through ray was traced */
+    uint32_t bin;             /* 0..7 azimuth sector, 8 zenith (dome.rs:304-310) */
+    float tp[OMG_MAX_BANDS];  /* per-band throughput delivered */
+    float dir[3];             /* departure direction */
+} omg_dome_ray;
+
+/* DomeProbe::sample without its EMA (dome.rs:425-459) for n_eyes listener
+ * positions: energy [n_eyes][9][nbands] and dsum [n_eyes][9][3] are the
+ * per-bin sums before the EMA; rays [n_eyes][n_rays] optional. */
+int omg_dome_sample(omg_scene *scene, const omg_dome_params *params, const float *eyes,
+                    uint32_t n_eyes, const omg_dome_panel *panels, uint32_t n_panels,
+                    float *energy, float *dsum, omg_dome_ray *rays, omg_counters *counters);
+
+/* Stateful probe = DomeProbe (dome.rs:293-302, :460-477): applies the 0.35
+ * EMA (first call primes) and the bin-direction normalisation.  out is
+ * [9][3 + nbands] rows of (dir[3], energy[nbands]). */
+typedef struct omg_dome_probe omg_dome_probe;
+int omg_dome_probe_create(omg_scene *scene, const omg_dome_params *params, omg_dome_probe **out);
+int omg_dome_probe_sample(omg_dome_probe *probe, const float eye[3],
+                          const omg_dome_panel *panels, uint32_t n_panels, float *out);
+void omg_dome_probe_destroy(omg_dome_probe *probe);
+
+/* ------------------------------------------------------------ image sources */
+
+/* ism::image_source_taps (crates/omg-core/src/ism.rs:15-84): taps as rows
+ * of (key, delay, dir[3], gains[nbands]).  *n_out receives the count. */
+int omg_ism_box(const float size[3], const omg_material walls[6], uint32_t nbands,
+                const float src[3], const float lis[3], uint32_t max_order, float *taps,
+                uint32_t cap, uint32_t *n_out);
+
+/* Overlay face table row for the solve (pt_mesh.rs:29-41) */
+typedef struct {
+    float n[3];
+    float d;
+    float refl[OMG_MAX_BANDS];
+    float trans[OMG_MAX_BANDS];
+    uint32_t has_rect;
+    float rmin[3], rmax[3];
+} omg_surface;
+
+/* Transient blocker for leg transmission (pt.rs:31-35, Aabb) */
+typedef struct {
+    float min[3], max[3];
+    float trans[OMG_MAX_BANDS];
+} omg_blocker;
+
+/* Solved path (pt_mesh.rs:111-120).  valid = 0: no record. */
+typedef struct {
+    uint32_t valid;
+    float delay;
+    float dir[3];
+    float gains[OMG_MAX_BANDS];
+} omg_mesh_record;
+
+/* SurfaceTable::build (pt_mesh.rs:51-80): surfaces of the scene's mesh;
+ * *n_out receives the count (the first overlay-face id). */
+int omg_surface_table(omg_scene *scene, omg_surface *out, uint32_t cap, uint32_t *n_out);
+
+/* WorldDiscovery::discover (crates/omg-scene/src/early_world.rs:24-26;
+ * mesh_chains, pt_mesh.rs:375-457) without overlay boxes: chains as
+ * [3] u16 + order, deduplicated in first-seen order. */
+int omg_discover(omg_scene *scene, const float listener[3], uint32_t n_rays, uint32_t rot,
+                 uint16_t *chains, uint8_t *orders, uint32_t cap, uint32_t *n_out);
+
+/* WorldSolve::solve_batch (early_world.rs:38-52; omg-gpu/src/lib.rs:510-609):
+ * records[si * n_chains + ci] for every (source, chain).  surfaces: the
+ * table of omg_surface_table plus any appended overlay faces. */
+int omg_solve_batch(omg_scene *scene, const omg_surface *surfaces, uint32_t n_surf,
+                    const float *sources, uint32_t n_src, const uint16_t *chains,
+                    const uint8_t *orders, uint32_t n_chains, const float listener[3],
+                    const omg_blocker *extras, uint32_t n_extras, const float *air_per_m,
+                    omg_mesh_record *records);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,566 @@
+// C ABI: lifecycle, scenes, the late-field tracer and its post-processing.
+#include <cmath>
+#include <cstring>
+
+#include "runtime.h"
+
+#pragma clang fp contract(off)
+
+namespace omg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace omg
+
+using namespace omg;
+
+extern "C" const char* omg_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ lifecycle
+
+extern "C" int omg_init(int device, uint32_t layout_version, omg_ctx** out)
+{
+    if (!out) return fail(OMG_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (layout_version != OMG_LAYOUT_VERSION)
+        return fail(OMG_ERR_VERSION, "layout version mismatch");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+        return fail(OMG_ERR_NO_DEVICE, "no usable HIP device");
+    OMG_HIP_CHECK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    OMG_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    omg_ctx* c = new omg_ctx();
+    c->device = device;
+    c->name = prop.gcnArchName;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(OMG_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return OMG_OK;
+}
+
+extern "C" void omg_shutdown(omg_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&c->job_mats, &c->job_ovl, &c->job_src, &c->job_lis, &c->job_bins,
+                      &c->job_dirs, &c->job_rec, &c->job_cnt, &c->aux0, &c->aux1, &c->aux2,
+                      &c->aux3, &c->aux4})
+        b->release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int omg_device_name(omg_ctx* c, char* buf, uint32_t len)
+{
+    if (!c || !buf || len == 0) return fail(OMG_ERR_INVALID, "bad argument");
+    std::strncpy(buf, c->name.c_str(), len - 1);
+    buf[len - 1] = 0;
+    return OMG_OK;
+}
+
+// ------------------------------------------------------------------ scenes
+
+int omg::upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s)
+{
+    OMG_HIP_CHECK(b.ensure(bytes ? bytes : 16));
+    if (bytes) OMG_HIP_CHECK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s));
+    return OMG_OK;
+}
+
+extern "C" int omg_scene_create(omg_ctx* c, const float* positions, uint32_t nv,
+                                const uint32_t* indices, uint32_t nt, const uint16_t* tri_material,
+                                const uint16_t* tri_surface, const float* tri_max_edge,
+                                const omg_material* materials, uint32_t nm, uint32_t nbands,
+                                omg_scene** out)
+{
+    if (!c || !out || (nt && (!positions || !indices || !tri_material)) || (nm && !materials))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    if (nbands < 1 || nbands > OMG_MAX_BANDS) return fail(OMG_ERR_INVALID, "nbands out of range");
+    for (uint32_t t = 0; t < nt; t++)
+        if (tri_material[t] >= nm) return fail(OMG_ERR_INVALID, "tri_material out of range");
+    *out = nullptr;
+    HostScene H;
+    std::string err;
+    if (build_scene(positions, nv, indices, nt, tri_material, tri_surface, tri_max_edge, H, err))
+        return fail(OMG_ERR_INVALID, err);
+    if (H.max_depth + 2 > (uint32_t)kStackCap)
+        return fail(OMG_ERR_CAP, "BVH deeper than the traversal stack");
+    OMG_HIP_CHECK(hipSetDevice(c->device));
+    omg_scene* s = new omg_scene();
+    s->ctx = c;
+    s->nbands = nbands;
+    s->n_tris = nt;
+    s->n_prims = (uint32_t)H.prims.size();
+    s->n_nodes = (uint32_t)H.nodes.size();
+    s->max_depth = H.max_depth;
+    s->root_ref = H.root_ref;
+    for (int k = 0; k < 3; k++) {
+        s->root_min[k] = H.root_min[k];
+        s->root_max[k] = H.root_max[k];
+    }
+    s->empty = H.empty;
+    s->materials.assign(materials, materials + nm);
+    s->mat_params.resize(nm);
+    for (uint32_t m = 0; m < nm; m++)
+        mat_params(materials[m].absorption, materials[m].scattering, materials[m].transmission,
+                   nbands, s->mat_params[m]);
+    s->tri_surface.assign(nt, 0);
+    if (tri_surface) s->tri_surface.assign(tri_surface, tri_surface + nt);
+    s->positions.assign(positions, positions + (size_t)nv * 3);
+    s->indices.assign(indices, indices + (size_t)nt * 3);
+    s->tri_material.assign(tri_material, tri_material + nt);
+    s->tris_host = H.tris;
+    int rc = upload(s->nodes, H.nodes.data(), H.nodes.size() * sizeof(Node2), c->stream);
+    if (!rc) rc = upload(s->prims, H.prims.data(), H.prims.size() * sizeof(Prim), c->stream);
+    if (!rc) rc = upload(s->tris, H.tris.data(), H.tris.size() * sizeof(TriInfo), c->stream);
+    if (!rc) rc = upload(s->mats, s->mat_params.data(), s->mat_params.size() * sizeof(MatParams), c->stream);
+    if (!rc) rc = upload(s->mats_raw, s->materials.data(), s->materials.size() * sizeof(omg_material), c->stream);
+    std::vector<uint32_t> ts32(s->tri_surface.begin(), s->tri_surface.end());
+    if (!rc) rc = upload(s->tri_surf, ts32.data(), ts32.size() * sizeof(uint32_t), c->stream);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "scene upload");
+    if (rc) {
+        s->nodes.release();
+        s->prims.release();
+        s->tris.release();
+        s->mats.release();
+        s->mats_raw.release();
+        s->tri_surf.release();
+        delete s;
+        return rc;
+    }
+    s->device_bytes = s->nodes.cap + s->prims.cap + s->tris.cap + s->mats.cap + s->mats_raw.cap +
+                      s->tri_surf.cap;
+    *out = s;
+    return OMG_OK;
+}
+
+extern "C" int omg_scene_create_soup(omg_ctx* c, const float* tri_verts, uint32_t nt,
+                                     const uint16_t* tri_material, const uint16_t* tri_surface,
+                                     const float* tri_max_edge, const omg_material* materials,
+                                     uint32_t nm, uint32_t nbands, omg_scene** out)
+{
+    if (nt && !tri_verts) return fail(OMG_ERR_INVALID, "tri_verts is null");
+    std::vector<float> pos;
+    std::vector<uint32_t> idx;
+    uint32_t nv = dedup_soup(tri_verts, nt, pos, idx);
+    return omg_scene_create(c, pos.data(), nv, idx.data(), nt, tri_material, tri_surface,
+                            tri_max_edge, materials, nm, nbands, out);
+}
+
+extern "C" int omg_scene_info_get(omg_scene* s, omg_scene_info* info)
+{
+    if (!s || !info) return fail(OMG_ERR_INVALID, "bad argument");
+    info->n_tris = s->n_tris;
+    info->n_prims = s->n_prims;
+    info->n_nodes = s->n_nodes;
+    info->max_depth = s->max_depth;
+    info->device_bytes = s->device_bytes;
+    return OMG_OK;
+}
+
+extern "C" int omg_dedup_soup(const float* tri_verts, uint32_t nt, float* positions_out,
+                              uint32_t* indices_out, uint32_t* nv_out)
+{
+    if ((nt && !tri_verts) || !nv_out) return fail(OMG_ERR_INVALID, "bad argument");
+    std::vector<float> pos;
+    std::vector<uint32_t> idx;
+    uint32_t nv = dedup_soup(tri_verts, nt, pos, idx);
+    if (positions_out) std::memcpy(positions_out, pos.data(), pos.size() * sizeof(float));
+    if (indices_out) std::memcpy(indices_out, idx.data(), idx.size() * sizeof(uint32_t));
+    *nv_out = nv;
+    return OMG_OK;
+}
+
+extern "C" int omg_bvh_build_host(const float* positions, uint32_t nv, const uint32_t* indices,
+                                  uint32_t nt, const uint16_t* tri_material,
+                                  const uint16_t* tri_surface, const float* tri_max_edge,
+                                  float* prims_out, uint32_t prim_cap, uint32_t* n_prims,
+                                  float* nodes_out, uint32_t node_cap, uint32_t* n_nodes,
+                                  uint32_t* root_ref, uint32_t* max_depth)
+{
+    if ((nt && (!positions || !indices || !tri_material)) || !n_prims || !n_nodes || !root_ref ||
+        !max_depth)
+        return fail(OMG_ERR_INVALID, "bad argument");
+    HostScene H;
+    std::string err;
+    if (build_scene(positions, nv, indices, nt, tri_material, tri_surface, tri_max_edge, H, err))
+        return fail(OMG_ERR_INVALID, err);
+    *n_prims = (uint32_t)H.prims.size();
+    *n_nodes = (uint32_t)H.nodes.size();
+    *root_ref = H.root_ref;
+    *max_depth = H.max_depth;
+    if (prims_out) {
+        if (prim_cap < H.prims.size()) return fail(OMG_ERR_CAP, "prim buffer too small");
+        std::memcpy(prims_out, H.prims.data(), H.prims.size() * sizeof(Prim));
+    }
+    if (nodes_out) {
+        if (node_cap < H.nodes.size()) return fail(OMG_ERR_CAP, "node buffer too small");
+        std::memcpy(nodes_out, H.nodes.data(), H.nodes.size() * sizeof(Node2));
+    }
+    return OMG_OK;
+}
+
+extern "C" void omg_scene_destroy(omg_scene* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    s->nodes.release();
+    s->prims.release();
+    s->tris.release();
+    s->mats.release();
+    s->mats_raw.release();
+    s->tri_surf.release();
+    delete s;
+}
+
+// ------------------------------------------------------------------ tracer
+
+namespace {
+
+uint32_t nbins_of(float max_time, float bin_dt) { return sat_u32(max_time / bin_dt); }
+
+int validate_params(const omg_trace_params* p, uint32_t nbands, uint32_t n_src, uint32_t n_lis,
+                    uint32_t n_ovl, uint32_t* nbins)
+{
+    if (!p) return fail(OMG_ERR_INVALID, "params is null");
+    if (p->nbands != nbands) return fail(OMG_ERR_INVALID, "params.nbands differs from the scene");
+    if (p->dir_band >= nbands) return fail(OMG_ERR_INVALID, "dir_band out of range");
+    if (p->n_rays == 0 || n_src == 0) return fail(OMG_ERR_INVALID, "no rays");
+    if (n_lis == 0) return fail(OMG_ERR_INVALID, "no listeners");
+    if (n_lis > OMG_MAX_LISTENERS) return fail(OMG_ERR_CAP, "too many listeners");
+    if (n_ovl > OMG_MAX_OVERLAYS) return fail(OMG_ERR_CAP, "too many overlays");
+    if ((uint64_t)p->n_rays * n_src >= (1ull << 32)) return fail(OMG_ERR_CAP, "too many rays");
+    if (p->max_bounces == 0 || p->max_bounces > 4096) return fail(OMG_ERR_INVALID, "max_bounces");
+    if (!(p->bin_dt > 0.0f) || !(p->max_time > 0.0f)) return fail(OMG_ERR_INVALID, "bin_dt/max_time");
+    *nbins = nbins_of(p->max_time, p->bin_dt);
+    if (*nbins == 0 || *nbins > (1u << 22)) return fail(OMG_ERR_INVALID, "echogram length");
+    return OMG_OK;
+}
+
+// Fills the launch-invariant fields and uploads the job's material table
+// (scene materials, then overlay materials) and overlays.
+int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatParams>& base_mats,
+                const omg_overlay* ovl, uint32_t n_ovl, uint32_t nbands, TraceArgs& A)
+{
+    std::vector<MatParams> mats(base_mats);
+    std::vector<Overlay> ov(n_ovl);
+    for (uint32_t i = 0; i < n_ovl; i++) {
+        MatParams mp;
+        mat_params(ovl[i].mat.absorption, ovl[i].mat.scattering, ovl[i].mat.transmission, nbands, mp);
+        mats.push_back(mp);
+        for (int k = 0; k < 3; k++) {
+            ov[i].mn[k] = ovl[i].min[k];
+            ov[i].mx[k] = ovl[i].max[k];
+        }
+        ov[i].mat = (uint32_t)(base_mats.size() + i);
+        ov[i].pad = 0;
+    }
+    int rc = upload(c->job_mats, mats.data(), mats.size() * sizeof(MatParams), c->stream);
+    if (!rc) rc = upload(c->job_ovl, ov.data(), ov.size() * sizeof(Overlay), c->stream);
+    // the staging vectors die with this frame: wait for the copies to land
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "job upload");
+    if (rc) return rc;
+    A.mats = c->job_mats.as<MatParams>();
+    A.overlays = c->job_ovl.as<Overlay>();
+    A.n_ovl = n_ovl;
+    A.n_rays = p->n_rays;
+    A.nbands = nbands;
+    A.max_bounces = p->max_bounces;
+    A.dir_band = p->dir_band;
+    A.key0 = (uint32_t)p->seed;
+    A.key1 = (uint32_t)(p->seed >> 32);
+    A.bin_dt = p->bin_dt;
+    A.max_time = p->max_time;
+    A.radius = p->radius;
+    float per_ray = 1.0f / (float)p->n_rays;
+    A.floor_e = p->floor_rel * per_ray;
+    for (int b = 0; b < kMaxBands; b++) A.energy0[b] = (uint32_t)b < nbands ? per_ray * p->energy[b] : 0.0f;
+    return OMG_OK;
+}
+
+int run_trace_host(omg_ctx* c, TraceArgs& A, bool mesh, uint32_t nbins, uint32_t nbands,
+                   const float* sources, uint32_t n_src, const float* listeners, uint32_t n_lis,
+                   float* bins, float* dirs, omg_ray_record* records, omg_counters* counters)
+{
+    size_t total = (size_t)A.n_rays * n_src;
+    size_t nb_f = (size_t)n_src * n_lis * nbins * nbands, nd_f = (size_t)n_src * n_lis * nbins * 3;
+    int rc = upload(c->job_src, sources, (size_t)n_src * 3 * sizeof(float), c->stream);
+    if (!rc) rc = upload(c->job_lis, listeners, (size_t)n_lis * 3 * sizeof(float), c->stream);
+    if (rc) return rc;
+    OMG_HIP_CHECK(c->job_bins.ensure(nb_f * sizeof(float)));
+    OMG_HIP_CHECK(c->job_dirs.ensure(nd_f * sizeof(float)));
+    if (records) OMG_HIP_CHECK(c->job_rec.ensure(total * sizeof(omg_ray_record)));
+    if (counters) OMG_HIP_CHECK(c->job_cnt.ensure(3 * sizeof(unsigned long long)));
+    A.sources = c->job_src.as<float>();
+    A.listeners = c->job_lis.as<float>();
+    A.n_src = n_src;
+    A.n_lis = n_lis;
+    A.nbins = nbins;
+    A.bins = c->job_bins.as<float>();
+    A.dirs = c->job_dirs.as<float>();
+    A.records = records ? c->job_rec.as<uint32_t>() : nullptr;
+    A.counters = counters ? c->job_cnt.as<unsigned long long>() : nullptr;
+    A.ray_base = 0;
+    A.ray_count = (uint32_t)total;
+    OMG_HIP_CHECK(launch_fill_zero(A.bins, nb_f, c->stream));
+    OMG_HIP_CHECK(launch_fill_zero(A.dirs, nd_f, c->stream));
+    if (counters) OMG_HIP_CHECK(launch_fill_zero_u64(A.counters, 3, c->stream));
+    OMG_HIP_CHECK(launch_trace(A, mesh, counters != nullptr, c->stream));
+    OMG_HIP_CHECK(hipMemcpyAsync(bins, A.bins, nb_f * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    OMG_HIP_CHECK(hipMemcpyAsync(dirs, A.dirs, nd_f * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (records)
+        OMG_HIP_CHECK(hipMemcpyAsync(records, A.records, total * sizeof(omg_ray_record),
+                                     hipMemcpyDeviceToHost, c->stream));
+    unsigned long long cnt[3] = {0, 0, 0};
+    if (counters)
+        OMG_HIP_CHECK(hipMemcpyAsync(cnt, A.counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    OMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (counters) {
+        counters->aabb_tests = cnt[0];
+        counters->tri_tests = cnt[1];
+        counters->bounce_rays = cnt[2];
+    }
+    return OMG_OK;
+}
+
+}  // namespace
+
+void omg::fill_geom(const omg_scene* s, GeomArgs& g)
+{
+    g.nodes = s->nodes.as<Node2>();
+    g.prims = s->prims.as<Prim>();
+    g.tris = s->tris.as<TriInfo>();
+    g.root_ref = s->root_ref;
+    g.empty = s->empty ? 1u : 0u;
+    g.box[0] = g.box[1] = g.box[2] = 0.0f;
+    g.pad = 0;
+    for (int k = 0; k < 3; k++) {
+        g.root_min[k] = s->root_min[k];
+        g.root_max[k] = s->root_max[k];
+    }
+}
+
+extern "C" int omg_trace_world(omg_scene* s, const omg_trace_params* p, const float* sources,
+                               uint32_t n_src, const float* listeners, uint32_t n_lis,
+                               const omg_overlay* overlays, uint32_t n_ovl, float* bins,
+                               float* dirs, omg_ray_record* records, omg_counters* counters)
+{
+    if (!s || !sources || !listeners || !bins || !dirs || (n_ovl && !overlays))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    uint32_t nbins;
+    int rc = validate_params(p, s->nbands, n_src, n_lis, n_ovl, &nbins);
+    if (rc) return rc;
+    omg_ctx* c = s->ctx;
+    OMG_HIP_CHECK(hipSetDevice(c->device));
+    TraceArgs A{};
+    fill_geom(s, A.g);
+    rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
+    if (rc) return rc;
+    return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
+                          dirs, records, counters);
+}
+
+extern "C" int omg_trace_box(omg_ctx* c, const float size[3], const omg_material walls[6],
+                             uint32_t nbands, const omg_trace_params* p, const float* sources,
+                             uint32_t n_src, const float* listeners, uint32_t n_lis, float* bins,
+                             float* dirs, omg_ray_record* records, omg_counters* counters)
+{
+    if (!c || !size || !walls || !sources || !listeners || !bins || !dirs)
+        return fail(OMG_ERR_INVALID, "bad argument");
+    if (nbands < 1 || nbands > OMG_MAX_BANDS) return fail(OMG_ERR_INVALID, "nbands out of range");
+    uint32_t nbins;
+    int rc = validate_params(p, nbands, n_src, n_lis, 0, &nbins);
+    if (rc) return rc;
+    OMG_HIP_CHECK(hipSetDevice(c->device));
+    std::vector<MatParams> wm(6);
+    for (int w = 0; w < 6; w++)
+        mat_params(walls[w].absorption, walls[w].scattering, walls[w].transmission, nbands, wm[w]);
+    TraceArgs A{};
+    A.g.nodes = nullptr;
+    A.g.prims = nullptr;
+    A.g.tris = nullptr;
+    A.g.root_ref = 0;
+    A.g.empty = 0;
+    for (int k = 0; k < 3; k++) A.g.box[k] = size[k];
+    rc = prepare_job(c, p, wm, nullptr, 0, nbands, A);
+    if (rc) return rc;
+    return run_trace_host(c, A, false, nbins, nbands, sources, n_src, listeners, n_lis, bins, dirs,
+                          records, counters);
+}
+
+extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
+                                     const float* d_sources, uint32_t n_src,
+                                     const float* d_listeners, uint32_t n_lis,
+                                     const omg_overlay* overlays, uint32_t n_ovl, float* d_bins,
+                                     float* d_dirs, omg_ray_record* d_records, void* stream)
+{
+    if (!s || !d_sources || !d_listeners || !d_bins || !d_dirs || (n_ovl && !overlays))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    uint32_t nbins;
+    int rc = validate_params(p, s->nbands, n_src, n_lis, n_ovl, &nbins);
+    if (rc) return rc;
+    omg_ctx* c = s->ctx;
+    OMG_HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    TraceArgs A{};
+    fill_geom(s, A.g);
+    // the job tables are staged on the context stream, then ordered before the launch
+    rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
+    if (rc) return rc;
+    if (st != c->stream) OMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    size_t nb_f = (size_t)n_src * n_lis * nbins * s->nbands, nd_f = (size_t)n_src * n_lis * nbins * 3;
+    A.sources = d_sources;
+    A.listeners = d_listeners;
+    A.n_src = n_src;
+    A.n_lis = n_lis;
+    A.nbins = nbins;
+    A.bins = d_bins;
+    A.dirs = d_dirs;
+    A.records = reinterpret_cast<uint32_t*>(d_records);
+    A.counters = nullptr;
+    A.ray_base = 0;
+    A.ray_count = p->n_rays * n_src;
+    OMG_HIP_CHECK(launch_fill_zero(d_bins, nb_f, st));
+    OMG_HIP_CHECK(launch_fill_zero(d_dirs, nd_f, st));
+    OMG_HIP_CHECK(launch_trace(A, true, false, st));
+    return OMG_OK;
+}
+
+// ------------------------------------------------------------------ post-processing (host)
+
+extern "C" int omg_estimate_reverb(const float* bins, uint32_t nbins, uint32_t nbands,
+                                   float bin_dt, float* rt60, float* level)
+{
+    if (!bins || !rt60 || !level || nbands == 0 || nbands > OMG_MAX_BANDS || !(bin_dt > 0.0f))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    std::vector<double> edc(nbins ? nbins : 1);
+    auto db = [&](size_t i, double total) {
+        double r = edc[i] / total;
+        return 10.0 * std::log10(r > 1e-12 ? r : 1e-12);
+    };
+    for (uint32_t b = 0; b < nbands; b++) {
+        rt60[b] = 0.5f;
+        double total = 0.0;
+        for (int64_t i = (int64_t)nbins - 1; i >= 0; i--) {
+            total += (double)bins[(size_t)i * nbands + b];
+            edc[i] = total;
+        }
+        if (total <= 0.0) continue;
+        int64_t t5 = -1, t25 = -1;
+        for (uint32_t i = 0; i < nbins; i++) {
+            double d = db(i, total);
+            if (t5 < 0 && d <= -5.0) t5 = i;
+            if (t25 < 0 && d <= -25.0) {
+                t25 = i;
+                break;
+            }
+        }
+        if (t5 >= 0 && t25 > t5) {
+            float dt = (float)(t25 - t5) * bin_dt;
+            float ddb = (float)(db(t25, total) - db(t5, total));
+            rt60[b] = fminf(fmaxf(-60.0f * dt / ddb, 0.1f), 5.0f);
+        }
+    }
+    uint32_t late_start = sat_u32(0.080f / bin_dt);
+    for (uint32_t b = 0; b < nbands; b++) {
+        float late_e = 0.0f;
+        for (uint32_t i = late_start; i < nbins; i++) late_e += bins[(size_t)i * nbands + b];
+        level[b] = fminf(sqrtf(late_e), 0.7f);
+    }
+    return OMG_OK;
+}
+
+extern "C" int omg_late_direction(const float* bins, const float* dirs, uint32_t nbins,
+                                  uint32_t nbands, float bin_dt, float from_s, float dir_out[3],
+                                  float* aniso)
+{
+    if (!bins || !dirs || !dir_out || !aniso || nbands < 2 || !(bin_dt > 0.0f))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    uint32_t start = sat_u32(from_s / bin_dt);
+    if (start > nbins) start = nbins;
+    float v[3] = {0.0f, 0.0f, 0.0f}, e = 0.0f;
+    for (uint32_t i = start; i < nbins; i++) {
+        for (int k = 0; k < 3; k++) v[k] += dirs[(size_t)i * 3 + k];
+        e += bins[(size_t)i * nbands + 1];
+    }
+    float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len < 1e-12f || e < 1e-12f) {
+        dir_out[0] = 1.0f;
+        dir_out[1] = 0.0f;
+        dir_out[2] = 0.0f;
+        *aniso = 0.0f;
+        return OMG_OK;
+    }
+    dir_out[0] = v[0] / len;
+    dir_out[1] = v[1] / len;
+    dir_out[2] = v[2] / len;
+    *aniso = fminf(len / e, 1.0f);
+    return OMG_OK;
+}
+
+// ism::image_source_taps (ism.rs:15-84), host-side closed form.
+extern "C" int omg_ism_box(const float size[3], const omg_material walls[6], uint32_t nb,
+                           const float src[3], const float lis[3], uint32_t max_order,
+                           float* taps, uint32_t cap, uint32_t* n_out)
+{
+    static const float kAir[3] = {1.0e-4f, 5.0e-4f, 3.0e-3f};   // material.rs:75
+    if (!size || !walls || !src || !lis || !n_out || nb == 0 || nb > OMG_MAX_BANDS || (cap && !taps))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    int n_max = ((int)max_order + 1) / 2 + 1;
+    float refl[6][OMG_MAX_BANDS];
+    for (int w = 0; w < 6; w++)
+        for (uint32_t b = 0; b < nb; b++) refl[w][b] = sqrtf(fmaxf(1.0f - walls[w].absorption[b], 0.0f));
+    f3 S = mk(src[0], src[1], src[2]), L = mk(lis[0], lis[1], lis[2]);
+    uint32_t n = 0;
+    const uint32_t row = 5 + nb;
+    for (int px = 0; px < 2; px++)
+        for (int py = 0; py < 2; py++)
+            for (int pz = 0; pz < 2; pz++)
+                for (int nx = -n_max; nx <= n_max; nx++)
+                    for (int ny = -n_max; ny <= n_max; ny++)
+                        for (int nz = -n_max; nz <= n_max; nz++) {
+                            uint32_t hits[6] = {(uint32_t)std::abs(nx - px), (uint32_t)std::abs(nx),
+                                                (uint32_t)std::abs(ny - py), (uint32_t)std::abs(ny),
+                                                (uint32_t)std::abs(nz - pz), (uint32_t)std::abs(nz)};
+                            uint32_t order = hits[0] + hits[1] + hits[2] + hits[3] + hits[4] + hits[5];
+                            if (order > max_order) continue;
+                            f3 img = mk((float)(1 - 2 * px) * S.x + 2.0f * (float)nx * size[0],
+                                        (float)(1 - 2 * py) * S.y + 2.0f * (float)ny * size[1],
+                                        (float)(1 - 2 * pz) * S.z + 2.0f * (float)nz * size[2]);
+                            f3 to_img = sub(img, L);
+                            float dist = fmaxf(length(to_img), 0.3f);
+                            f3 dir = normalize(to_img);
+                            if (n < cap) {
+                                float* o = taps + (size_t)n * row;
+                                o[0] = (float)n;
+                                o[1] = dist / kSpeedOfSound;
+                                o[2] = dir.x;
+                                o[3] = dir.y;
+                                o[4] = dir.z;
+                                for (uint32_t b = 0; b < nb; b++) {
+                                    float g = expf(-kAir[b < 3 ? b : 2] * dist) / dist;
+                                    for (int w = 0; w < 6; w++)
+                                        for (uint32_t k = 0; k < hits[w]; k++) g *= refl[w][b];
+                                    o[5 + b] = g;
+                                }
+                            }
+                            n++;
+                        }
+    *n_out = n < cap ? n : cap;
+    if (n > cap) return fail(OMG_ERR_CAP, "tap buffer too small");
+    return OMG_OK;
+}

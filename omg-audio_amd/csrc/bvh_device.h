@@ -1,0 +1,246 @@
+// Device-side BVH queries shared by the tracer, dome and image-source kernels.
+//
+// All traversals walk the paired 64 B node layout and replay the reference's
+// visiting order exactly (crates/omg-core/src/mesh.rs:484-604), so every
+// per-ray decision matches the CPU oracle bit for bit.
+#pragma once
+#include "trace_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace omg {
+
+// Mesh::ray_aabb (mesh.rs:409-426) on one box; t_enter is the entry distance.
+__device__ __forceinline__ bool ray_aabb(const float bmin[3], const float bmax[3], f3 o, f3 inv,
+                                         float tmax, float& t_enter)
+{
+    float t0 = 0.0f, t1 = tmax;
+    float ta = (bmin[0] - o.x) * inv.x, tb = (bmax[0] - o.x) * inv.x;
+    if (ta > tb) { float s = ta; ta = tb; tb = s; }
+    t0 = fmaxf(t0, ta);
+    t1 = fminf(t1, tb);
+    if (t0 > t1) return false;
+    ta = (bmin[1] - o.y) * inv.y;
+    tb = (bmax[1] - o.y) * inv.y;
+    if (ta > tb) { float s = ta; ta = tb; tb = s; }
+    t0 = fmaxf(t0, ta);
+    t1 = fminf(t1, tb);
+    if (t0 > t1) return false;
+    ta = (bmin[2] - o.z) * inv.z;
+    tb = (bmax[2] - o.z) * inv.z;
+    if (ta > tb) { float s = ta; ta = tb; tb = s; }
+    t0 = fmaxf(t0, ta);
+    t1 = fminf(t1, tb);
+    if (t0 > t1) return false;
+    t_enter = t0;
+    return true;
+}
+
+// Mesh::ray_tri_packed (mesh.rs:437-456): two-sided Moller-Trumbore.
+__device__ __forceinline__ bool ray_tri(const Prim& p, f3 o, f3 d, float& t_out, float& u_out,
+                                        float& v_out)
+{
+    f3 e1 = mk(p.e1[0], p.e1[1], p.e1[2]);
+    f3 e2 = mk(p.e2[0], p.e2[1], p.e2[2]);
+    f3 pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (fabsf(det) < 1e-9f) return false;
+    float inv = 1.0f / det;
+    f3 s = sub(o, mk(p.a[0], p.a[1], p.a[2]));
+    float u = dot(s, pv) * inv;
+    if (!(u >= -1e-6f && u <= 1.0f + 1e-6f)) return false;
+    f3 q = cross(s, e1);
+    float v = dot(d, q) * inv;
+    if (v < -1e-6f || u + v > 1.0f + 1e-6f) return false;
+    float t = dot(e2, q) * inv;
+    if (!(t > 1e-5f)) return false;
+    t_out = t;
+    u_out = u;
+    v_out = v;
+    return true;
+}
+
+__device__ __forceinline__ Prim load_prim(const Prim* prims, uint32_t k)
+{
+    const float4* q = reinterpret_cast<const float4*>(prims + k);
+    float4 a = q[0], b = q[1], c = q[2];
+    Prim p;
+    p.a[0] = a.x; p.a[1] = a.y; p.a[2] = a.z; p.tri = __float_as_uint(a.w);
+    p.e1[0] = b.x; p.e1[1] = b.y; p.e1[2] = b.z; p.mat = __float_as_uint(b.w);
+    p.e2[0] = c.x; p.e2[1] = c.y; p.e2[2] = c.z; p.surf = __float_as_uint(c.w);
+    return p;
+}
+
+struct TravCount {
+    uint32_t aabb = 0, tri = 0;
+};
+
+// Mesh::raycast (mesh.rs:484-541): ordered traversal, near child popped
+// first (ties to the left child), strict `t < best`, pop-time cull on
+// `t_enter >= best`.  flags: bit0 winning hit within 1e-6 barycentric of
+// an edge, bit1 exact-t tie with another original triangle.
+template <bool COUNT>
+__device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, float& t_out, uint32_t& tri_out,
+                             uint32_t& flags, TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    uint32_t st_ref[kStackCap];
+    float st_t[kStackCap];
+    int sp = 1;
+    st_ref[0] = G.root_ref;
+    st_t[0] = 0.0f;
+    bool found = false;
+    float best = FLT_MAX, bu = 0.0f, bv = 0.0f;
+    uint32_t btri = 0, tie = 0;
+    while (sp > 0) {
+        sp--;
+        uint32_t ref = st_ref[sp];
+        float te = st_t[sp];
+        if (found && te >= best) continue;
+        if (ref & kLeafBit) {
+            uint32_t start = ref & kLeafStartMask;
+            uint32_t cnt_p = (ref >> kLeafCountShift) & 7u;
+            for (uint32_t k = start; k < start + cnt_p; k++) {
+                Prim p = load_prim(G.prims, k);
+                if (COUNT) cnt.tri++;
+                float t, u, v;
+                if (ray_tri(p, o, d, t, u, v)) {
+                    if (!found || t < best) {
+                        found = true;
+                        best = t;
+                        btri = p.tri;
+                        bu = u;
+                        bv = v;
+                        tie = 0;
+                    } else if (t == best && p.tri != btri) {
+                        tie = 1;
+                    }
+                }
+            }
+        } else {
+            const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
+            float4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
+            const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
+            const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
+            uint32_t lref = __float_as_uint(n0.w), rref = __float_as_uint(n1.w);
+            float tmax = found ? best : FLT_MAX;
+            float a = 0.0f, b = 0.0f;
+            if (COUNT) cnt.aabb += 2;
+            bool hl = ray_aabb(lmin, lmax, o, inv, tmax, a);
+            bool hr = ray_aabb(rmin, rmax, o, inv, tmax, b);
+            if (hl && hr) {
+                bool lnear = a <= b;
+                st_ref[sp] = lnear ? rref : lref;
+                st_t[sp] = lnear ? b : a;
+                st_ref[sp + 1] = lnear ? lref : rref;
+                st_t[sp + 1] = lnear ? a : b;
+                sp += 2;
+            } else if (hl) {
+                st_ref[sp] = lref;
+                st_t[sp] = a;
+                sp += 1;
+            } else if (hr) {
+                st_ref[sp] = rref;
+                st_t[sp] = b;
+                sp += 1;
+            }
+        }
+    }
+    if (!found) return false;
+    t_out = best;
+    tri_out = btri;
+    flags = ((bu < 1e-6f || bv < 1e-6f || bu + bv > 1.0f - 1e-6f) ? 1u : 0u) | (tie ? 2u : 0u);
+    return true;
+}
+
+
+// One crossing of a segment: position along it (0..1), original triangle,
+// authored surface and material.
+struct SegHit {
+    float t;
+    uint32_t tri;
+    uint32_t surf;
+    uint32_t mat;
+};
+
+// Mesh::segment_hits (mesh.rs:546-604) into a caller-provided list of
+// capacity `cap`: all crossings strictly inside the open segment, stably
+// sorted by t, then each same-t cluster reduced to its lowest surface id.
+// The reference pops a node, tests its box and pushes both children (the
+// second child is visited first); here a child's box is tested before it
+// is pushed, which visits the same leaves in the same order.  Returns the
+// count, or -1 when more than `cap` crossings were found.
+template <bool COUNT>
+__device__ __forceinline__ int segment_hits(const GeomArgs& G, f3 a, f3 b, SegHit* out, int cap,
+                                            TravCount& cnt)
+{
+    if (G.empty) return 0;
+    f3 d = sub(b, a);
+    float len = length(d);
+    if (len < 1e-6f) return 0;
+    f3 dn = mul(d, 1.0f / len);
+    f3 inv = mk(1.0f / dn.x, 1.0f / dn.y, 1.0f / dn.z);
+    int n = 0;
+    bool overflow = false;
+    uint32_t stack[kStackCap];
+    int sp = 0;
+    float te;
+    if (COUNT) cnt.aabb++;
+    if (ray_aabb(G.root_min, G.root_max, a, inv, len, te)) stack[sp++] = G.root_ref;
+    while (sp > 0) {
+        uint32_t ref = stack[--sp];
+        if (ref & kLeafBit) {
+            uint32_t start = ref & kLeafStartMask;
+            uint32_t cnt_p = (ref >> kLeafCountShift) & 7u;
+            for (uint32_t k = start; k < start + cnt_p; k++) {
+                Prim p = load_prim(G.prims, k);
+                if (COUNT) cnt.tri++;
+                float t, u, v;
+                if (ray_tri(p, a, dn, t, u, v)) {
+                    float tt = t / len;
+                    if (tt > 1e-4f && tt < 1.0f - 1e-4f) {
+                        if (n == cap) {
+                            overflow = true;
+                            continue;
+                        }
+                        // stable insertion: after every entry with t <= tt
+                        int j = n;
+                        while (j > 0 && out[j - 1].t > tt) {
+                            out[j] = out[j - 1];
+                            j--;
+                        }
+                        out[j] = SegHit{tt, p.tri, p.surf, p.mat};
+                        n++;
+                    }
+                }
+            }
+        } else {
+            const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
+            float4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
+            const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
+            const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
+            if (COUNT) cnt.aabb += 2;
+            if (ray_aabb(lmin, lmax, a, inv, len, te)) stack[sp++] = __float_as_uint(n0.w);
+            if (ray_aabb(rmin, rmax, a, inv, len, te)) stack[sp++] = __float_as_uint(n1.w);
+        }
+    }
+    if (overflow) return -1;
+    // same-t dedup, lowest surface id wins (mesh.rs:589-603)
+    float tol = 1e-4f / fmaxf(len, 1.0f);
+    int w = 0;
+    for (int i = 0; i < n; i++) {
+        if (w > 0 && fabsf(out[i].t - out[w - 1].t) < tol) {
+            if (out[i].surf < out[w - 1].surf) {
+                out[w - 1].tri = out[i].tri;
+                out[w - 1].surf = out[i].surf;
+                out[w - 1].mat = out[i].mat;
+            }
+        } else {
+            out[w++] = out[i];
+        }
+    }
+    return w;
+}
+
+}  // namespace omg

@@ -1,0 +1,75 @@
+// Device-resident data layout of a scene and of one tracer launch.
+//
+// BVH: the reference's binary SAH tree (mesh.rs:292-405) flattened into
+// 64-byte "paired" nodes: an internal node stores BOTH children's boxes and
+// references, so one cache-line-aligned 64 B fetch serves the two AABB
+// tests the reference performs when it pops an internal node
+// (mesh.rs:510-537).  Leaves are not stored as nodes: a child reference
+// with bit 31 set encodes the leaf's primitive range directly.
+#pragma once
+#include <stdint.h>
+
+namespace omg {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafCountShift = 28;          // count (1..4) in bits 28..30
+constexpr uint32_t kLeafStartMask = 0x0FFFFFFFu;  // start in bits 0..27
+constexpr int kMaxBands = 8;
+constexpr int kStackCap = 64;                     // the reference's stack (mesh.rs:492)
+
+inline __host__ __device__ uint32_t leaf_ref(uint32_t start, uint32_t count)
+{
+    return kLeafBit | (count << kLeafCountShift) | start;
+}
+
+struct alignas(64) Node2 {
+    float lmin[3];
+    uint32_t lref;
+    float lmax[3];
+    uint32_t rref;
+    float rmin[3];
+    uint32_t pad0;
+    float rmax[3];
+    uint32_t pad1;
+};
+static_assert(sizeof(Node2) == 64, "Node2 is one 64 B line");
+
+// Packed primitive (mesh.rs:43-50): vertex a, edges e1 = b - a, e2 = c - a,
+// the ORIGINAL triangle id and its material.  48 B = three 16 B loads.
+struct alignas(16) Prim {
+    float a[3];
+    uint32_t tri;
+    float e1[3];
+    uint32_t mat;
+    float e2[3];
+    uint32_t surf;
+};
+static_assert(sizeof(Prim) == 48, "Prim is 48 B");
+
+// Per original triangle: unit normal of normalize(cross(b-a, c-a))
+// (mesh.rs:458-461) and the material index.
+struct alignas(16) TriInfo {
+    float n[3];
+    uint32_t mat;
+};
+
+// Per-material tracer constants precomputed on the host with the
+// reference's f32 expressions (tracer.rs:148-185): the transmission
+// probability p, scattering, the per-band transmitted weight (t*t)/p and
+// the reflected weight keep/(1-p).  The kernel multiplies by these, which
+// is bit-identical to evaluating the reference's ratios per bounce.
+struct MatParams {
+    float p;
+    float scattering;
+    float tw[kMaxBands];
+    float rw[kMaxBands];
+};
+
+struct Overlay {
+    float mn[3];
+    float mx[3];
+    uint32_t mat;   // index into the launch's material table
+    uint32_t pad;
+};
+
+}  // namespace omg

@@ -1,0 +1,85 @@
+// Host runtime internals shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/omg_hip.h"
+#include "omg_layout.h"
+#include "scene_build.h"
+#include "trace_kernels.h"
+
+namespace omg {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define OMG_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::omg::fail(OMG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        size_t want = bytes < 4096 ? 4096 : bytes + bytes / 4;
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
+}  // namespace omg
+
+struct omg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    // per-call staging (one user thread at a time per context)
+    omg::DevBuf job_mats, job_ovl, job_src, job_lis, job_bins, job_dirs, job_rec, job_cnt;
+    omg::DevBuf aux0, aux1, aux2, aux3, aux4;
+};
+
+struct omg_scene {
+    omg_ctx* ctx = nullptr;
+    uint32_t nbands = 3;
+    uint32_t n_tris = 0, n_prims = 0, n_nodes = 0, max_depth = 0, root_ref = 0;
+    float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
+    bool empty = false;
+    std::vector<omg_material> materials;       // host copy (overlay tables append to it)
+    std::vector<omg::MatParams> mat_params;    // tracer constants of the scene materials
+    std::vector<uint16_t> tri_surface;         // host copy for chain ids
+    std::vector<float> positions;              // host copy for the surface table
+    std::vector<uint32_t> indices;
+    std::vector<uint16_t> tri_material;
+    std::vector<omg::TriInfo> tris_host;
+    omg::DevBuf nodes, prims, tris, mats, mats_raw, tri_surf;
+    uint64_t device_bytes = 0;
+};
+
+namespace omg {
+hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s);
+hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s);
+hipError_t launch_fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
+int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s);
+void fill_geom(const omg_scene* s, GeomArgs& g);
+}  // namespace omg

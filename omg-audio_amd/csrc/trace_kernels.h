@@ -1,0 +1,42 @@
+// Device traversal and tracer kernels (wave64, gfx950).
+#pragma once
+#include "omg_layout.h"
+#include "omg_numerics.h"
+
+namespace omg {
+
+// Geometry of one launch: a flattened mesh BVH, or the analytic Shoebox.
+struct GeomArgs {
+    const Node2* nodes;
+    const Prim* prims;
+    const TriInfo* tris;
+    uint32_t root_ref;
+    uint32_t empty;          // mesh without primitives: every ray escapes
+    float box[3];            // Shoebox size (box launches)
+    uint32_t pad;
+    float root_min[3];       // bounds of the whole tree (the root's own box)
+    float root_max[3];
+};
+
+struct TraceArgs {
+    GeomArgs g;
+    const MatParams* mats;   // scene materials, then overlay materials (box: 6 walls first)
+    const Overlay* overlays;
+    uint32_t n_ovl;
+    const float* sources;    // [n_src][3]
+    const float* listeners;  // [n_lis][3]
+    uint32_t n_src, n_lis;
+    uint32_t n_rays, nbands, max_bounces, dir_band;
+    uint32_t key0, key1;     // Philox key = job seed
+    float bin_dt, max_time, floor_e, radius;
+    float energy0[kMaxBands];  // per_ray * source energy (tracer.rs:95)
+    uint32_t nbins;
+    float* bins;             // [n_src][n_lis][nbins][nbands]
+    float* dirs;             // [n_src][n_lis][nbins][3]
+    uint32_t* records;       // [n_src * n_rays][4] or null
+    unsigned long long* counters;  // [3] or null
+    uint32_t ray_base;       // first global ray of this launch (chunked launches)
+    uint32_t ray_count;      // rays in this launch
+};
+
+}  // namespace omg

@@ -1,0 +1,163 @@
+"""ctypes binding of libomg_hip.so (the C ABI declared in include/omg_hip.h).
+
+This is the binding a host language adds over the C ABI; the Rust
+equivalent is shown in INTEGRATION.md.  Loading fails loudly when the
+library has not been built: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "libomg_hip.so")
+
+LAYOUT_VERSION = 1
+MAX_BANDS = 8
+MAX_OVERLAYS = 64
+MAX_LISTENERS = 64
+DOME_BINS = 9
+
+OK = 0
+ERR_NO_DEVICE = -1
+ERR_INVALID = -2
+ERR_CAP = -3
+ERR_VERSION = -4
+ERR_DEVICE = -5
+ERR_BUDGET = -6
+ERR_NOT_READY = -7
+ERR_OVERFLOW = -8
+
+F8 = C.c_float * MAX_BANDS
+
+
+class Material(C.Structure):
+    _fields_ = [("absorption", F8), ("scattering", C.c_float), ("transmission", F8)]
+
+
+class Overlay(C.Structure):
+    _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3), ("mat", Material)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("n_rays", C.c_uint32), ("nbands", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("dir_band", C.c_uint32), ("seed", C.c_uint64), ("bin_dt", C.c_float),
+                ("max_time", C.c_float), ("floor_rel", C.c_float), ("radius", C.c_float),
+                ("energy", F8)]
+
+
+class RayRecord(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("term", C.c_uint32), ("hit_hash", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("aabb_tests", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("bounce_rays", C.c_uint64)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_tris", C.c_uint32), ("n_prims", C.c_uint32), ("n_nodes", C.c_uint32),
+                ("max_depth", C.c_uint32), ("device_bytes", C.c_uint64)]
+
+
+class DomePanel(C.Structure):
+    _fields_ = [("axis", C.c_uint32), ("plane", C.c_float), ("lat0", C.c_float),
+                ("lat1", C.c_float), ("z0", C.c_float), ("z1", C.c_float),
+                ("glass", C.c_uint32), ("trans", F8)]
+
+
+class DomeParams(C.Structure):
+    _fields_ = [("n_fan", C.c_uint32), ("n_rays", C.c_uint32), ("max_events", C.c_uint32),
+                ("nbands", C.c_uint32), ("horizon", C.c_float), ("through_len", C.c_float)]
+
+
+class DomeRay(C.Structure):
+    _fields_ = [("escaped", C.c_uint32), ("events", C.c_uint32), ("used_through", C.c_uint32),
+                ("bin", C.c_uint32), ("tp", F8), ("dir", C.c_float * 3)]
+
+
+class Surface(C.Structure):
+    _fields_ = [("n", C.c_float * 3), ("d", C.c_float), ("refl", F8), ("trans", F8),
+                ("has_rect", C.c_uint32), ("rmin", C.c_float * 3), ("rmax", C.c_float * 3)]
+
+
+class Blocker(C.Structure):
+    _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3), ("trans", F8)]
+
+
+class MeshRecord(C.Structure):
+    _fields_ = [("valid", C.c_uint32), ("delay", C.c_float), ("dir", C.c_float * 3),
+                ("gains", F8)]
+
+
+class BatchQuery(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", TraceParams), ("sources", C.c_void_p),
+                ("n_src", C.c_uint32), ("listeners", C.c_void_p), ("n_lis", C.c_uint32),
+                ("bins", C.c_void_p), ("dirs", C.c_void_p)]
+
+
+# (name, restype, argtypes) of every exported entry point
+_vp, _u32, _f32, _i = C.c_void_p, C.c_uint32, C.c_float, C.c_int
+SIGNATURES = [
+    ("omg_init", _i, [_i, _u32, C.POINTER(_vp)]),
+    ("omg_shutdown", None, [_vp]),
+    ("omg_last_error", C.c_char_p, []),
+    ("omg_device_name", _i, [_vp, C.c_char_p, _u32]),
+    ("omg_scene_create", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32,
+                              C.POINTER(_vp)]),
+    ("omg_scene_create_soup", _i, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32,
+                                   C.POINTER(_vp)]),
+    ("omg_scene_info_get", _i, [_vp, _vp]),
+    ("omg_dedup_soup", _i, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
+    ("omg_bvh_build_host", _i, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32,
+                                C.POINTER(_u32), _vp, _u32, C.POINTER(_u32), C.POINTER(_u32),
+                                C.POINTER(_u32)]),
+    ("omg_scene_destroy", None, [_vp]),
+    ("omg_trace_world", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("omg_trace_box", _i, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("omg_trace_world_async", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp,
+                                   _vp]),
+    ("omg_estimate_reverb", _i, [_vp, _u32, _u32, _f32, _vp, _vp]),
+    ("omg_late_direction", _i, [_vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp]),
+    ("omg_dome_sample", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("omg_dome_probe_create", _i, [_vp, _vp, C.POINTER(_vp)]),
+    ("omg_dome_probe_sample", _i, [_vp, _vp, _vp, _u32, _vp]),
+    ("omg_dome_probe_destroy", None, [_vp]),
+    ("omg_ism_box", _i, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, C.POINTER(_u32)]),
+    ("omg_surface_table", _i, [_vp, _vp, _u32, C.POINTER(_u32)]),
+    ("omg_discover", _i, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, C.POINTER(_u32)]),
+    ("omg_solve_batch", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp,
+                             _vp]),
+]
+
+_LIB = None
+
+
+class OmgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"omg_hip error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libomg_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run omg-audio_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().omg_last_error()
+        raise OmgError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,248 @@
+"""Seeded procedural scenes for the tracer workloads.
+
+``MeshBuilder`` mirrors the reference's authoring API (material, surface,
+tri, quad, abox, coarse; crates/omg-core/src/mesh.rs:70-165) and emits a
+triangle soup; the library applies the 0.1 mm vertex dedup and builds the
+BVH.  All randomness is SplitMix64 from an explicit seed, so every scene
+is reproducible bit for bit.
+"""
+import math
+
+import numpy as np
+
+from . import MaterialDef
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): the scene-synthesis stream."""
+
+    def __init__(self, seed):
+        self.state = seed & MASK64
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+    def log_uniform(self, lo, hi):
+        return math.exp(self.uniform(math.log(lo), math.log(hi)))
+
+    def randint(self, lo, hi):
+        """Uniform integer in [lo, hi]."""
+        return lo + self.next_u64() % (hi - lo + 1)
+
+
+class MeshBuilder:
+    """Triangle-soup builder with the reference's authoring calls (mesh.rs:88-153)."""
+
+    def __init__(self):
+        self.verts = []
+        self.tri_material = []
+        self.tri_surface = []
+        self.tri_max_edge = []
+        self.materials = []
+        self.next_surface = 0
+        self.cur_max_edge = 0.0
+
+    def material(self, m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def begin_surface(self):
+        self.next_surface += 1
+        return self.next_surface - 1
+
+    def coarse(self, max_edge):
+        self.cur_max_edge = max_edge
+
+    def tri(self, a, b, c, material):
+        self.verts.extend((a, b, c))
+        self.tri_material.append(material)
+        self.tri_surface.append(max(self.next_surface - 1, 0))
+        self.tri_max_edge.append(self.cur_max_edge if self.cur_max_edge > 0 else 4.0)
+
+    def quad(self, a, b, c, d, material):
+        self.tri(a, b, c, material)
+        self.tri(a, c, d, material)
+
+    def abox(self, mn, mx, material):
+        """mesh.rs:144-153: six outward-wound faces."""
+        (ax, ay, az), (bx, by, bz) = mn, mx
+        self.quad((ax, ay, az), (ax, by, az), (ax, by, bz), (ax, ay, bz), material)
+        self.quad((bx, ay, az), (bx, ay, bz), (bx, by, bz), (bx, by, az), material)
+        self.quad((ax, ay, az), (ax, ay, bz), (bx, ay, bz), (bx, ay, az), material)
+        self.quad((ax, by, az), (bx, by, az), (bx, by, bz), (ax, by, bz), material)
+        self.quad((ax, ay, az), (bx, ay, az), (bx, by, az), (ax, by, az), material)
+        self.quad((ax, ay, bz), (ax, by, bz), (bx, by, bz), (bx, ay, bz), material)
+
+    @property
+    def n_tris(self):
+        return len(self.tri_material)
+
+    def soup(self):
+        return dict(tri_verts=np.asarray(self.verts, np.float32).reshape(-1, 9),
+                    tri_material=np.asarray(self.tri_material, np.uint16),
+                    tri_surface=np.asarray(self.tri_surface, np.uint16),
+                    tri_max_edge=np.asarray(self.tri_max_edge, np.float32),
+                    materials=list(self.materials))
+
+
+def _rand_material(rng, nbands, scattering):
+    absorption = [rng.uniform(0.02, 0.4) for _ in range(nbands)]
+    return MaterialDef(absorption, scattering, [0.0] * nbands)
+
+
+def shoebox(seed=1, size=(12.0, 8.0, 4.0), nbands=3, scattering=0.3, clearance=1.0):
+    """Config: one shoebox room as 12 triangles with per-surface random absorption.
+
+    Faces follow MeshBuilder::abox's winding (mesh.rs:147-152); each face is
+    its own surface with absorption U[0.02, 0.4] per band, scattering 0.3,
+    transmission 0.  Returns (builder, source, listener, wall materials in
+    Shoebox order x-, x+, y-, y+, z-, z+).
+    """
+    rng = SplitMix64(seed)
+    mats = [_rand_material(rng, nbands, scattering) for _ in range(6)]
+    mb = MeshBuilder()
+    ids = [mb.material(m) for m in mats]
+    (ax, ay, az), (bx, by, bz) = (0.0, 0.0, 0.0), size
+    faces = [
+        ((ax, ay, az), (ax, by, az), (ax, by, bz), (ax, ay, bz)),
+        ((bx, ay, az), (bx, ay, bz), (bx, by, bz), (bx, by, az)),
+        ((ax, ay, az), (ax, ay, bz), (bx, ay, bz), (bx, ay, az)),
+        ((ax, by, az), (bx, by, az), (bx, by, bz), (ax, by, bz)),
+        ((ax, ay, az), (bx, ay, az), (bx, by, az), (ax, by, az)),
+        ((ax, ay, bz), (ax, by, bz), (bx, by, bz), (bx, ay, bz)),
+    ]
+    for f, m in zip(faces, ids):
+        mb.begin_surface()
+        mb.quad(*f, m)
+
+    def inside():
+        return tuple(rng.uniform(clearance, s - clearance) for s in size)
+
+    src = inside()
+    lis = inside()
+    return mb, np.asarray(src, np.float32), np.asarray(lis, np.float32), mats
+
+
+# four material classes of the city block: (absorption range, scattering, transmission)
+_CLASSES = [
+    ((0.02, 0.06), 0.12, [0.045, 0.008, 0.0012]),   # brick / masonry
+    ((0.01, 0.05), 0.10, [0.10, 0.014, 0.0015]),    # concrete
+    ((0.05, 0.25), 0.15, [0.22, 0.060, 0.0200]),    # wood / panel
+    ((0.10, 0.40), 0.40, [0.30, 0.100, 0.0300]),    # soft / planted
+]
+
+
+def _class_materials(rng, nbands):
+    mats = []
+    for (lo, hi), scat, trans in _CLASSES:
+        absorption = [rng.uniform(lo, hi) for _ in range(nbands)]
+        tr = [trans[min(b * 3 // nbands, 2)] for b in range(nbands)]
+        mats.append(MaterialDef(absorption, scat, tr))
+    return mats
+
+
+def _wall_cells(mb, mat, plane_axis, plane, lo, hi, z0, z1, cell, holes):
+    """A vertical wall as a grid of quads with rectangular holes removed.
+
+    plane_axis 0: wall in plane x = plane spanning y in [lo, hi];
+    plane_axis 1: plane y = plane spanning x.  holes: (u0, u1, v0, v1).
+    """
+    nu = max(1, int(math.ceil((hi - lo) / cell)))
+    nv = max(1, int(math.ceil((z1 - z0) / cell)))
+    du = (hi - lo) / nu
+    dv = (z1 - z0) / nv
+
+    def P(u, v):
+        return (plane, u, v) if plane_axis == 0 else (u, plane, v)
+
+    for i in range(nu):
+        u0, u1 = lo + i * du, lo + (i + 1) * du
+        um = 0.5 * (u0 + u1)
+        for j in range(nv):
+            v0, v1 = z0 + j * dv, z0 + (j + 1) * dv
+            vm = 0.5 * (v0 + v1)
+            if any(h[0] <= um <= h[1] and h[2] <= vm <= h[3] for h in holes):
+                continue
+            mb.quad(P(u0, v0), P(u1, v0), P(u1, v1), P(u0, v1), mat)
+
+
+def city_block(seed=2, n_buildings=40, target_tris=50_000, nbands=3, extent=None,
+               side=(3.0, 15.0), height=(3.0, 12.0)):
+    """Config: procedural city block of extruded boxes with door/window holes.
+
+    Buildings: sides U[3, 15] m, heights U[3, 12] m, 1-3 holes per wall
+    (door 1 x 2.1 m or window 1.2 x 1.2 m), walls and roofs gridded so the
+    block totals about target_tris triangles; 4 material classes with
+    seeded absorption; a ground plane (tessellated to 4 m by the BVH build).
+    Returns (builder, bounds (min, max), building boxes).
+    """
+    rng = SplitMix64(seed)
+    mats = _class_materials(rng, nbands)
+    if extent is None:
+        extent = max(60.0, 12.0 * math.sqrt(n_buildings))
+    boxes = []
+    for _ in range(n_buildings):
+        sx, sy = rng.uniform(*side), rng.uniform(*side)
+        h = rng.uniform(*height)
+        x0 = rng.uniform(0.0, extent - sx)
+        y0 = rng.uniform(0.0, extent - sy)
+        boxes.append(((x0, y0, 0.0), (x0 + sx, y0 + sy, h), rng.randint(0, 3)))
+    # total authored area -> grid cell size hitting the triangle target
+    area = sum(2 * ((b[1][0] - b[0][0]) + (b[1][1] - b[0][1])) * b[1][2] +
+               (b[1][0] - b[0][0]) * (b[1][1] - b[0][1]) for b in boxes)
+    cell = max(0.25, math.sqrt(2.0 * area / max(1, target_tris - 2)))
+    mb = MeshBuilder()
+    ids = [mb.material(m) for m in mats]
+    mb.begin_surface()
+    g = extent * 0.25
+    mb.quad((-g, -g, 0.0), (extent + g, -g, 0.0), (extent + g, extent + g, 0.0),
+            (-g, extent + g, 0.0), ids[1])
+    for (x0, y0, _), (x1, y1, h), mi in boxes:
+        m = ids[mi]
+        walls = [(0, x0, y0, y1), (0, x1, y0, y1), (1, y0, x0, x1), (1, y1, x0, x1)]
+        for axis, plane, lo, hi in walls:
+            holes = []
+            for _ in range(rng.randint(1, 3)):
+                door = rng.uniform() < 0.4
+                w, hh = (1.0, 2.1) if door else (1.2, 1.2)
+                if hi - lo < w + 0.4 or h < hh + 0.4:
+                    continue
+                u0 = rng.uniform(lo + 0.2, hi - 0.2 - w)
+                v0 = 0.0 if door else rng.uniform(0.6, min(h - hh - 0.2, 0.6 + 2.5 * h))
+                holes.append((u0, u0 + w, v0, v0 + hh))
+            mb.begin_surface()
+            _wall_cells(mb, m, axis, plane, lo, hi, 0.0, h, cell, holes)
+        mb.begin_surface()
+        nx = max(1, int(math.ceil((x1 - x0) / cell)))
+        ny = max(1, int(math.ceil((y1 - y0) / cell)))
+        for i in range(nx):
+            for j in range(ny):
+                ax, bx = x0 + (x1 - x0) * i / nx, x0 + (x1 - x0) * (i + 1) / nx
+                ay, by = y0 + (y1 - y0) * j / ny, y0 + (y1 - y0) * (j + 1) / ny
+                mb.quad((ax, ay, h), (bx, ay, h), (bx, by, h), (ax, by, h), m)
+    bounds = (np.array([0.0, 0.0, 0.0], np.float32), np.array([extent, extent, 12.0], np.float32))
+    return mb, bounds, [(b[0], b[1]) for b in boxes]
+
+
+def outdoor_points(seed, n, bounds, boxes, z=(1.2, 1.8), margin=0.6):
+    """n points at ear height inside bounds and outside every building box."""
+    rng = SplitMix64(seed)
+    out = []
+    lo, hi = bounds
+    while len(out) < n:
+        p = (rng.uniform(lo[0] + 2, hi[0] - 2), rng.uniform(lo[1] + 2, hi[1] - 2),
+             rng.uniform(*z))
+        if any(b0[0] - margin < p[0] < b1[0] + margin and b0[1] - margin < p[1] < b1[1] + margin
+               for b0, b1 in boxes):
+            continue
+        out.append(p)
+    return np.asarray(out, np.float32)

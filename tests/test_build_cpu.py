@@ -1,0 +1,135 @@
+"""CPU tests of the library's host side: scene build equals the oracle's,
+and the C ABI exports every declared symbol."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from omg_hip import _lib, scenes
+from helpers import oracle_mesh_from_builder
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "omg_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?(omg_\w+)\s*\(", text,
+                                 re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    bound = {s[0] for s in _lib.SIGNATURES}
+    assert not [s for s in syms if s not in bound], "binding lacks declared entry points"
+
+
+def _host_build(mb):
+    L = _lib.lib()
+    soup = mb.soup()
+    tv = np.ascontiguousarray(soup["tri_verts"])
+    nt = tv.shape[0]
+    pos = np.zeros((nt * 3, 3), np.float32)
+    idx = np.zeros((nt, 3), np.uint32)
+    nv = C.c_uint32()
+    _lib.check(L.omg_dedup_soup(tv.ctypes.data, nt, pos.ctypes.data, idx.ctypes.data,
+                                C.byref(nv)))
+    pos = pos[:nv.value].copy()
+    npr, nn, root, depth = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    args = [pos.ctypes.data, nv.value, idx.ctypes.data, nt, soup["tri_material"].ctypes.data,
+            soup["tri_surface"].ctypes.data, soup["tri_max_edge"].ctypes.data]
+    _lib.check(L.omg_bvh_build_host(*args, None, 0, C.byref(npr), None, 0, C.byref(nn),
+                                    C.byref(root), C.byref(depth)))
+    prims = np.zeros((npr.value, 12), np.float32)
+    nodes = np.zeros((max(1, nn.value), 16), np.float32)
+    _lib.check(L.omg_bvh_build_host(*args, prims.ctypes.data, npr.value, C.byref(npr),
+                                    nodes.ctypes.data, nn.value, C.byref(nn), C.byref(root),
+                                    C.byref(depth)))
+    return pos, idx, prims, nodes[:nn.value], root.value, depth.value
+
+
+@pytest.mark.parametrize("which", ["shoebox", "city", "soup"])
+def test_scene_build_matches_oracle(which, oracle):
+    if which == "shoebox":
+        mb = scenes.shoebox(seed=1)[0]
+    elif which == "city":
+        mb = scenes.city_block(seed=2, n_buildings=8, target_tris=6000)[0]
+    else:
+        # random triangle soup as in mesh.rs:697-715 (tri edges up to 4 m: tessellation kicks in)
+        rng = scenes.SplitMix64(7)
+        mb = scenes.MeshBuilder()
+        m = mb.material(scenes.MaterialDef([0.1] * 3, 0.1, [0.0] * 3))
+        for _ in range(200):
+            b = [rng.uniform(0, 20) for _ in range(3)]
+            mb.tri(tuple(b), tuple(x + rng.uniform(-3, 3) for x in b),
+                   tuple(x + rng.uniform(-3, 3) for x in b), m)
+    pos, idx, prims, nodes, root, depth = _host_build(mb)
+    om = oracle_mesh_from_builder(mb)
+    opos, oidx = O.dedup_tris(mb.soup()["tri_verts"])
+    assert np.array_equal(pos, opos) and np.array_equal(idx, oidx)
+    onodes, oprims = om.export()
+    # primitives: identical bits and order (a, e1, e2, tri, material)
+    pv = prims.view(np.uint32)
+    ov = oprims.view(np.uint32)
+    assert pv.shape[0] == ov.shape[0]
+    for a, b in [((0, 1, 2), (0, 1, 2)), ((4, 5, 6), (3, 4, 5)), ((8, 9, 10), (6, 7, 8)),
+                 ((3,), (9,)), ((7,), (10,))]:
+        assert np.array_equal(pv[:, list(a)], ov[:, list(b)])
+    # paired nodes: walk both trees together and compare child boxes and leaf ranges
+    on_u = onodes.view(np.uint32)
+
+    def oref(i):
+        a, b = on_u[i, 6], on_u[i, 7]
+        if a & 0x80000000:
+            return ("leaf", int(a & 0x7FFFFFFF), int(b))
+        return ("node", i)
+
+    def pref(r):
+        if r & 0x80000000:
+            return ("leaf", int(r & 0x0FFFFFFF), int((r >> 28) & 7))
+        return ("node", int(r))
+
+    stack = [(0, root)]
+    seen = 0
+    while stack:
+        oi, r = stack.pop()
+        o = oref(oi)
+        p = pref(r)
+        assert o[0] == p[0]
+        if o[0] == "leaf":
+            assert o[1:] == p[1:]
+            continue
+        seen += 1
+        n = nodes[p[1]]
+        nu = n.view(np.uint32)
+        la, lb = int(on_u[oi, 6]), int(on_u[oi, 7])
+        assert np.array_equal(n[0:3], onodes[la, 0:3]) and np.array_equal(n[4:7], onodes[la, 3:6])
+        assert np.array_equal(n[8:11], onodes[lb, 0:3]) and np.array_equal(n[12:15], onodes[lb, 3:6])
+        stack.append((la, int(nu[3])))
+        stack.append((lb, int(nu[7])))
+    assert seen == nodes.shape[0]
+
+
+def test_host_postprocessing_matches_oracle(oracle):
+    """estimate_reverb / late_direction / ism_box run on the host: same bits as the oracle."""
+    import omg_hip as H
+    rng = np.random.default_rng(3)
+    t = np.arange(300) * 0.01
+    bins = (np.exp(-t / 0.2)[:, None] * rng.uniform(0.5, 1.5, (300, 3))).astype(np.float32)
+    dirs = rng.normal(size=(300, 3)).astype(np.float32) * bins[:, 1:2]
+    assert all(np.array_equal(a, b) for a, b in zip(H.estimate_reverb(bins, 0.01),
+                                                    O.estimate_reverb(bins, 0.01)))
+    d1, a1 = H.late_direction(bins, dirs, 0.01, 0.05)
+    d2, a2 = O.late_direction(bins, dirs, 0.01, 0.05)
+    assert np.array_equal(d1, d2) and a1 == a2
+    walls = [H.DRYWALL] * 6
+    taps = H.ism_box((8, 6, 3), walls, (2, 2, 1.5), (5, 4, 1.5), 3)
+    ref = O.image_source_taps((8, 6, 3), [(m.absorption, m.scattering, m.transmission)
+                                          for m in walls], (2, 2, 1.5), (5, 4, 1.5), 3)
+    assert taps.shape == (63, 8) and np.array_equal(taps, ref)
