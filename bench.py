@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: bounce-rays/s of the stochastic energy path tracer on one MI355X.
+
+Workload (the plain single-scene launch): a seeded ~50k-triangle city block
+of 40 extruded buildings with door and window holes; 16 sources x 4096
+rays x 3 bands (half the sources indoors), each ray scored against 8
+listeners (1 ms x 2 s echograms, 64 bounces, relative floor 1e-7); plus
+the 8 listeners' 512-ray Fibonacci dome fans against a 500 m sky sphere.
+One step = one tracer launch + one dome launch over that batch, inputs and
+outputs resident in HBM.  `value` counts tracer bounce-rays (iterations of
+the reference's bounce loop, crates/omg-core/src/tracer.rs:103) plus dome
+ray-events (escape events and straight-through segments).
+
+Prints ONE JSON line.  --gpus above 1 is refused: this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "omg-audio_amd"))
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+NODE_BYTES = 32           # bytes per AABB test (one child box, crates/omg-gpu/src/layout.rs:96-107)
+PRIM_BYTES = 48           # bytes per triangle test (layout.rs:109-122)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="minimum wall time of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(mb, src, lis, n_rays, seed, max_seconds):
+    """The oracle (C port of the reference tracer) on the host cores, same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import numpy as np
+    soup = mb.soup()
+    pos, idx = O.dedup_tris(soup["tri_verts"])
+    mats = [(list(m.absorption), m.scattering, list(m.transmission)) for m in soup["materials"]]
+    om = O.Mesh(pos, idx, soup["tri_material"], mats, 3, soup["tri_surface"], soup["tri_max_edge"])
+    g = O.mesh_geom(om)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    params = [O.trace_params(n_rays, rng_mode=O.RNG_COUNTER, seed=seed, src_index=i,
+                             bin_dt=0.001, max_time=2.0) for i in range(src.shape[0])]
+    reps, br, t0 = 0, 0, time.perf_counter()
+    while True:
+        _, stats = O.trace_jobs_threaded(g, src, lis, params, threads)
+        br += int(stats[:, 2].sum())
+        reps += 1
+        if time.perf_counter() - t0 >= max_seconds:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=br / dt, unit="bounce-rays/s", cores=threads, kind="port",
+                sample=f"the full tracer workload ({src.shape[0]} sources x {n_rays} rays x "
+                       f"{lis.shape[0]} listeners) repeated {reps}x in {dt:.1f} s, "
+                       f"one job per source on a {threads}-thread pool, counter RNG mode")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the path shards by source/query with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import omg_hip as H
+    from omg_hip import scenes
+
+    dev = torch.device("cuda", 0)
+    ctx = H.Context(0)
+    mb, src, lis = scenes.c2_workload()
+    scene = H.Scene.from_builder(ctx, mb)
+    info = scene.info()
+    n_rays, seed = 4096, 0xC2
+    params = H.trace_params(n_rays, 3, seed=seed, bin_dt=0.001, max_time=2.0)
+    dparams = H.dome_params(3, n_fan=512, horizon=500.0)
+    n_src, n_lis = src.shape[0], lis.shape[0]
+    nbins = H.nbins(2.0, 0.001)
+
+    # one counted pass: traversal work of exactly the timed launch
+    counted = H.trace_world(scene, params, src, lis, counters=True)
+    tc = counted.counters
+    dome = H.dome_sample(scene, lis, dparams, rays=True, counters=True)
+    dome_events = int(dome["events"].sum() + dome["through"].sum())
+    alg_bytes_trace = NODE_BYTES * tc["aabb_tests"] + PRIM_BYTES * tc["tri_tests"]
+
+    d_src = torch.from_numpy(src).to(dev)
+    d_lis = torch.from_numpy(lis).to(dev)
+    d_bins = torch.empty((n_src, n_lis, nbins, 3), dtype=torch.float32, device=dev)
+    d_dirs = torch.empty((n_src, n_lis, nbins, 3), dtype=torch.float32, device=dev)
+    d_energy = torch.empty((n_lis, 9, 3), dtype=torch.float32, device=dev)
+    d_dsum = torch.empty((n_lis, 9, 3), dtype=torch.float32, device=dev)
+    d_rays = torch.empty((n_lis * dparams.n_rays, 16), dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    sh = stream.cuda_stream
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        H.trace_world_async(scene, params, d_src.data_ptr(), n_src, d_lis.data_ptr(), n_lis,
+                            d_bins.data_ptr(), d_dirs.data_ptr(), stream=sh)
+        if ev is not None:
+            ev[1].record(stream)
+        H.dome_sample_async(scene, d_lis.data_ptr(), n_lis, d_energy.data_ptr(),
+                            d_dsum.data_ptr(), d_rays.data_ptr(), dparams, stream=sh)
+        if ev is not None:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(args.steps):
+        step(evs[k])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    trace_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    dome_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    H.dome_status(scene)
+
+    # the timed launch reproduces the counted pass (same seed, same inputs)
+    got = d_bins.cpu().numpy()
+    rel = float(np.max(np.abs(got - counted.bins)) / max(float(np.max(np.abs(counted.bins))), 1e-30))
+
+    ms_per_step = total_ms / args.steps
+    work = tc["bounce_rays"] + dome_events
+    value = work / (ms_per_step * 1e-3)
+    achieved = alg_bytes_trace / (trace_ms * 1e-3) / 1e9
+    out = {
+        "metric": "bounce-rays/s (plain single-scene launch)",
+        "value": value,
+        "unit": "bounce-rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural city block)",
+        "config": {
+            "workload": "city block ~50k tris, 16 sources x 4096 rays x 3 bands scored at 8 "
+                        "listeners (1 ms x 2 s), + 8 x 512-ray dome fans, 500 m sky",
+            "tris": info["n_tris"], "prims": info["n_prims"], "bvh_nodes": info["n_nodes"],
+            "sources": n_src, "rays_per_source": n_rays, "listeners": n_lis, "bands": 3,
+            "parallelism": "single GPU",
+        },
+        "tracer": {"bounce_rays_per_step": tc["bounce_rays"], "ms": trace_ms,
+                   "bounce_rays_per_s": tc["bounce_rays"] / (trace_ms * 1e-3),
+                   "aabb_tests": tc["aabb_tests"], "tri_tests": tc["tri_tests"]},
+        "dome": {"ray_events_per_step": dome_events, "ms": dome_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_trace", "alg_bytes_per_launch": alg_bytes_trace},
+        "wall_s": wall,
+        "timed_vs_counted_rel_err": rel,
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -225,6 +225,18 @@ int omg_dome_sample(omg_scene *scene, const omg_dome_params *params, const float
                     uint32_t n_eyes, const omg_dome_panel *panels, uint32_t n_panels,
                     float *energy, float *dsum, omg_dome_ray *rays, omg_counters *counters);
 
+/* Device-resident variant: d_eyes, d_energy, d_dsum and d_rays (scratch of
+ * n_eyes * n_rays omg_dome_ray, required) are device pointers; the launch
+ * is enqueued on `stream` (NULL = the context's stream) without waiting.
+ * Panels are host memory, copied before the launch.  A crossing-list
+ * overflow is reported by the next omg_dome_status call. */
+int omg_dome_sample_async(omg_scene *scene, const omg_dome_params *params, const float *d_eyes,
+                          uint32_t n_eyes, const omg_dome_panel *panels, uint32_t n_panels,
+                          float *d_energy, float *d_dsum, omg_dome_ray *d_rays, void *stream);
+/* Waits for the context's dome work and returns OMG_ERR_OVERFLOW if any
+ * asynchronous dome launch since the last call overflowed, else OMG_OK. */
+int omg_dome_status(omg_scene *scene);
+
 /* Stateful probe = DomeProbe (dome.rs:293-302, :460-477): applies the 0.35
  * EMA (first call primes) and the bin-direction normalisation.  out is
  * [9][3 + nbands] rows of (dir[3], energy[nbands]). */

@@ -123,6 +123,79 @@ extern "C" int omg_dome_sample(omg_scene* s, const omg_dome_params* p, const flo
     return OMG_OK;
 }
 
+extern "C" int omg_dome_sample_async(omg_scene* s, const omg_dome_params* p, const float* d_eyes,
+                                     uint32_t n_eyes, const omg_dome_panel* panels,
+                                     uint32_t n_panels, float* d_energy, float* d_dsum,
+                                     omg_dome_ray* d_rays, void* stream)
+{
+    int rc = validate(s, p, n_eyes, n_panels);
+    if (rc) return rc;
+    if (!d_eyes || !d_energy || !d_dsum || !d_rays || (n_panels && !panels))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    if (n_eyes == 0) return OMG_OK;
+    omg_ctx* c = s->ctx;
+    OMG_HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (c->fan_n != p->n_fan || !c->dome_status.ptr) {
+        std::vector<float> fan;
+        std::vector<uint32_t> fan_bin;
+        make_fan(p->n_fan, fan, fan_bin);
+        rc = upload(c->fan_dirs, fan.data(), fan.size() * sizeof(float), c->stream);
+        if (!rc) rc = upload(c->fan_bins, fan_bin.data(), fan_bin.size() * sizeof(uint32_t), c->stream);
+        if (rc) return rc;
+        if (!c->dome_status.ptr) {
+            OMG_HIP_CHECK(c->dome_status.ensure(sizeof(uint32_t)));
+            OMG_HIP_CHECK(launch_fill_zero((float*)c->dome_status.ptr, 1, c->stream));
+        }
+        OMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+        c->fan_n = p->n_fan;
+    }
+    if (n_panels) {
+        rc = upload(c->dome_panels, panels, n_panels * sizeof(omg_dome_panel), c->stream);
+        if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "panels");
+        if (rc) return rc;
+    } else {
+        OMG_HIP_CHECK(c->dome_panels.ensure(sizeof(omg_dome_panel)));
+    }
+    DomeArgs A{};
+    fill_geom(s, A.g);
+    A.mats = s->mats_raw.as<omg_material_dev>();
+    A.panels = c->dome_panels.as<DomePanelDev>();
+    A.n_panels = n_panels;
+    A.eyes = d_eyes;
+    A.n_eyes = n_eyes;
+    A.fan = c->fan_dirs.as<float>();
+    A.fan_bin = c->fan_bins.as<uint32_t>();
+    A.n_fan = p->n_fan;
+    A.n_rays = p->n_rays;
+    A.max_events = p->max_events;
+    A.nbands = p->nbands;
+    A.horizon = p->horizon;
+    A.through_len = p->through_len;
+    A.rays = reinterpret_cast<DomeRayDev*>(d_rays);
+    A.energy = d_energy;
+    A.dsum = d_dsum;
+    A.counters = nullptr;
+    A.status = c->dome_status.as<uint32_t>();
+    OMG_HIP_CHECK(launch_dome(A, false, st));
+    return OMG_OK;
+}
+
+extern "C" int omg_dome_status(omg_scene* s)
+{
+    if (!s) return fail(OMG_ERR_INVALID, "bad argument");
+    omg_ctx* c = s->ctx;
+    if (!c->dome_status.ptr) return OMG_OK;
+    OMG_HIP_CHECK(hipSetDevice(c->device));
+    OMG_HIP_CHECK(hipDeviceSynchronize());
+    uint32_t v = 0;
+    OMG_HIP_CHECK(hipMemcpy(&v, c->dome_status.ptr, sizeof v, hipMemcpyDeviceToHost));
+    OMG_HIP_CHECK(launch_fill_zero((float*)c->dome_status.ptr, 1, c->stream));
+    OMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (v & 1u) return fail(OMG_ERR_OVERFLOW, "a straight-through ray crossed too many surfaces");
+    return OMG_OK;
+}
+
 extern "C" int omg_dome_probe_create(omg_scene* s, const omg_dome_params* p, omg_dome_probe** out)
 {
     if (!out) return fail(OMG_ERR_INVALID, "out is null");

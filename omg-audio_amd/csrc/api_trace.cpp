@@ -50,17 +50,26 @@ extern "C" int omg_init(int device, uint32_t layout_version, omg_ctx** out)
     return OMG_OK;
 }
 
-extern "C" void omg_shutdown(omg_ctx* c)
+static void ctx_teardown(omg_ctx* c)
 {
-    if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->job_mats, &c->job_ovl, &c->job_src, &c->job_lis, &c->job_bins,
                       &c->job_dirs, &c->job_rec, &c->job_cnt, &c->aux0, &c->aux1, &c->aux2,
-                      &c->aux3, &c->aux4})
+                      &c->aux3, &c->aux4, &c->fan_dirs, &c->fan_bins, &c->dome_status,
+                      &c->dome_panels})
         b->release();
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+// The context outlives its scenes whatever order the caller releases them in:
+// with scenes alive, the teardown waits for the last omg_scene_destroy.
+extern "C" void omg_shutdown(omg_ctx* c)
+{
+    if (!c) return;
+    c->closing = true;
+    if (c->live_scenes == 0) ctx_teardown(c);
 }
 
 extern "C" int omg_device_name(omg_ctx* c, char* buf, uint32_t len)
@@ -143,6 +152,7 @@ extern "C" int omg_scene_create(omg_ctx* c, const float* positions, uint32_t nv,
     }
     s->device_bytes = s->nodes.cap + s->prims.cap + s->tris.cap + s->mats.cap + s->mats_raw.cap +
                       s->tri_surf.cap;
+    c->live_scenes++;
     *out = s;
     return OMG_OK;
 }
@@ -224,7 +234,10 @@ extern "C" void omg_scene_destroy(omg_scene* s)
     s->mats.release();
     s->mats_raw.release();
     s->tri_surf.release();
+    omg_ctx* c = s->ctx;
     delete s;
+    c->live_scenes--;
+    if (c->closing && c->live_scenes == 0) ctx_teardown(c);
 }
 
 // ------------------------------------------------------------------ tracer

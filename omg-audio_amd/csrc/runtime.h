@@ -57,6 +57,12 @@ struct omg_ctx {
     // per-call staging (one user thread at a time per context)
     omg::DevBuf job_mats, job_ovl, job_src, job_lis, job_bins, job_dirs, job_rec, job_cnt;
     omg::DevBuf aux0, aux1, aux2, aux3, aux4;
+    // dome fan of the last size used, and the sticky overflow word of async launches
+    omg::DevBuf fan_dirs, fan_bins, dome_status, dome_panels;
+    uint32_t fan_n = 0;
+    // scenes alive on this context; omg_shutdown defers the teardown until the last is destroyed
+    uint32_t live_scenes = 0;
+    bool closing = false;
 };
 
 struct omg_scene {

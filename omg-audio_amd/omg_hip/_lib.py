@@ -120,6 +120,8 @@ SIGNATURES = [
     ("omg_estimate_reverb", _i, [_vp, _u32, _u32, _f32, _vp, _vp]),
     ("omg_late_direction", _i, [_vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp]),
     ("omg_dome_sample", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("omg_dome_sample_async", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("omg_dome_status", _i, [_vp]),
     ("omg_dome_probe_create", _i, [_vp, _vp, C.POINTER(_vp)]),
     ("omg_dome_probe_sample", _i, [_vp, _vp, _vp, _u32, _vp]),
     ("omg_dome_probe_destroy", None, [_vp]),

@@ -68,6 +68,23 @@ def dome_sample(scene, eyes, params=None, panels=(), rays=False, counters=False)
     return out
 
 
+def dome_sample_async(scene, d_eyes, n_eyes, d_energy, d_dsum, d_rays, params=None, panels=(),
+                      stream=None):
+    """Device-resident dome pass: pointers are device addresses (ints), d_rays
+    is scratch for n_eyes * n_rays ray records; enqueued on `stream`."""
+    params = params or dome_params(scene.nbands)
+    pa = make_panels(list(panels))
+    check(lib().omg_dome_sample_async(scene.h, C.byref(params), C.c_void_p(d_eyes), n_eyes,
+                                      C.cast(pa, C.c_void_p), len(panels), C.c_void_p(d_energy),
+                                      C.c_void_p(d_dsum), C.c_void_p(d_rays),
+                                      C.c_void_p(stream) if stream else None))
+
+
+def dome_status(scene):
+    """Raises OmgError(ERR_OVERFLOW) if an async dome pass overflowed a crossing list."""
+    check(lib().omg_dome_status(scene.h))
+
+
 class DomeProbe:
     """Stateful probe: the 0.35 EMA across calls (dome.rs:460-477)."""
 

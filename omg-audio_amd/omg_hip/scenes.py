@@ -233,6 +233,37 @@ def city_block(seed=2, n_buildings=40, target_tris=50_000, nbands=3, extent=None
     return mb, bounds, [(b[0], b[1]) for b in boxes]
 
 
+def indoor_points(seed, n, boxes, z=(1.2, 1.8), margin=0.5):
+    """n points at ear height inside seeded building boxes (rejecting points
+    inside any other building's footprint margin band)."""
+    rng = SplitMix64(seed)
+    out = []
+    while len(out) < n:
+        (x0, y0, _), (x1, y1, h) = boxes[rng.randint(0, len(boxes) - 1)]
+        if h < z[1] + 0.5:
+            continue
+        p = (rng.uniform(x0 + margin, x1 - margin), rng.uniform(y0 + margin, y1 - margin),
+             rng.uniform(*z))
+        near_wall = any(
+            (b0[0] - margin < p[0] < b1[0] + margin and b0[1] - margin < p[1] < b1[1] + margin) and
+            not (b0[0] + margin < p[0] < b1[0] - margin and b0[1] + margin < p[1] < b1[1] - margin)
+            for b0, b1 in boxes)
+        if not near_wall:
+            out.append(p)
+    return np.asarray(out, np.float32)
+
+
+def c2_workload(seed=2):
+    """Plain single-launch workload: the ~50k-triangle city block, 16 sources
+    (half inside buildings, half in the streets), 8 listeners (likewise)."""
+    mb, bounds, boxes = city_block(seed=seed)
+    src = np.vstack([indoor_points(seed * 100 + 1, 8, boxes),
+                     outdoor_points(seed * 100 + 2, 8, bounds, boxes)])
+    lis = np.vstack([indoor_points(seed * 100 + 3, 4, boxes),
+                     outdoor_points(seed * 100 + 4, 4, bounds, boxes)])
+    return mb, src, lis
+
+
 def outdoor_points(seed, n, bounds, boxes, z=(1.2, 1.8), margin=0.6):
     """n points at ear height inside bounds and outside every building box."""
     rng = SplitMix64(seed)
