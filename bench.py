@@ -105,22 +105,33 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     sh = stream.cuda_stream
 
+    # the dome fans run on a second stream, overlapped with the tracer launch
+    stream2 = torch.cuda.Stream(device=dev)
+    sh2 = stream2.cuda_stream
+
     def step(ev=None):
+        fork = torch.cuda.Event()
+        fork.record(stream)
+        stream2.wait_event(fork)
         if ev is not None:
             ev[0].record(stream)
         H.trace_world_async(scene, params, d_src.data_ptr(), n_src, d_lis.data_ptr(), n_lis,
                             d_bins.data_ptr(), d_dirs.data_ptr(), stream=sh)
         if ev is not None:
             ev[1].record(stream)
+            ev[2].record(stream2)
         H.dome_sample_async(scene, d_lis.data_ptr(), n_lis, d_energy.data_ptr(),
-                            d_dsum.data_ptr(), d_rays.data_ptr(), dparams, stream=sh)
+                            d_dsum.data_ptr(), d_rays.data_ptr(), dparams, stream=sh2)
         if ev is not None:
-            ev[2].record(stream)
+            ev[3].record(stream2)
+        join = torch.cuda.Event()
+        join.record(stream2)
+        stream.wait_event(join)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -133,7 +144,7 @@ def main():
     wall = time.perf_counter() - wall0
     total_ms = t_start.elapsed_time(t_end)
     trace_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    dome_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    dome_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in evs]))
     H.dome_status(scene)
 
     # the timed launch reproduces the counted pass (same seed, same inputs)

@@ -108,6 +108,11 @@ void omg_shutdown(omg_ctx *ctx);
 /* Thread-local message for the last failing call on this thread. */
 const char *omg_last_error(void);
 int omg_device_name(omg_ctx *ctx, char *buf, uint32_t len);
+/* Tracer launch shape (no reference counterpart; set by the autotuner):
+ * BVH nodes of the top levels staged in LDS per block, and traversal-stack
+ * entries per lane kept in LDS (16 or 32; deeper entries spill to scratch).
+ * Results do not depend on it. */
+int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds);
 
 /* ------------------------------------------------------------ scenes */
 

@@ -72,6 +72,17 @@ extern "C" void omg_shutdown(omg_ctx* c)
     if (c->live_scenes == 0) ctx_teardown(c);
 }
 
+extern "C" int omg_set_launch_config(omg_ctx* c, uint32_t node_cache, uint32_t stack_lds)
+{
+    if (!c) return fail(OMG_ERR_INVALID, "bad argument");
+    if (stack_lds != 16 && stack_lds != 32) return fail(OMG_ERR_INVALID, "stack_lds must be 16 or 32");
+    size_t lds = (size_t)node_cache * sizeof(Node2) + (size_t)stack_lds * 256 * 8;
+    if (lds > 160 * 1024) return fail(OMG_ERR_CAP, "LDS working set above 160 KiB");
+    c->node_cache = node_cache;
+    c->stack_lds = stack_lds;
+    return OMG_OK;
+}
+
 extern "C" int omg_device_name(omg_ctx* c, char* buf, uint32_t len)
 {
     if (!c || !buf || len == 0) return fail(OMG_ERR_INVALID, "bad argument");
@@ -381,6 +392,8 @@ extern "C" int omg_trace_world(omg_scene* s, const omg_trace_params* p, const fl
     OMG_HIP_CHECK(hipSetDevice(c->device));
     TraceArgs A{};
     fill_geom(s, A.g);
+    A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
+    A.stack_lds = c->stack_lds;
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
@@ -431,6 +444,8 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     TraceArgs A{};
     fill_geom(s, A.g);
+    A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
+    A.stack_lds = c->stack_lds;
     // the job tables are staged on the context stream, then ordered before the launch
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;

@@ -155,6 +155,122 @@ __device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, floa
 }
 
 
+// LDS working set of one block for mesh_raycast_lds: a prefix of the node
+// array (the top levels, nodes are stored breadth-first) and the top SK
+// entries of every lane's traversal stack, laid out [depth][lane] so that
+// lanes at the same depth hit distinct banks.
+struct LdsTrav {
+    const float4* nodes;   // [nc][4]
+    uint32_t nc;
+    uint32_t* st_ref;      // [SK][bs]
+    float* st_t;           // [SK][bs]
+    uint32_t lane, bs;
+};
+
+__device__ __forceinline__ void load_node(const GeomArgs& G, const LdsTrav& L, uint32_t ref,
+                                          float4& n0, float4& n1, float4& n2, float4& n3)
+{
+    if (ref < L.nc) {
+        const float4* q = L.nodes + 4 * ref;
+        n0 = q[0]; n1 = q[1]; n2 = q[2]; n3 = q[3];
+    } else {
+        const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
+        n0 = q[0]; n1 = q[1]; n2 = q[2]; n3 = q[3];
+    }
+}
+
+// Mesh::raycast (mesh.rs:484-541) with the stack top in LDS.  The near child
+// becomes the current node directly instead of being pushed and popped at
+// once; its pop-time cull (`t_enter >= best`) is applied in the same place,
+// so the visiting order, the culls and the result are the reference's.
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_lds(const GeomArgs& G, const LdsTrav& L, f3 o, f3 d,
+                                                 float& t_out, uint32_t& tri_out, uint32_t& flags,
+                                                 TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    uint32_t ovf_ref[kStackCap - SK];
+    float ovf_t[kStackCap - SK];
+    int sp = 0;
+    uint32_t cur = G.root_ref;
+    float cur_t = 0.0f;
+    bool found = false;
+    float best = FLT_MAX, bu = 0.0f, bv = 0.0f;
+    uint32_t btri = 0, tie = 0;
+    while (true) {
+        if (!(found && cur_t >= best)) {
+            if (cur & kLeafBit) {
+                uint32_t start = cur & kLeafStartMask;
+                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+                for (uint32_t k = start; k < start + cnt_p; k++) {
+                    Prim p = load_prim(G.prims, k);
+                    if (COUNT) cnt.tri++;
+                    float t, u, v;
+                    if (ray_tri(p, o, d, t, u, v)) {
+                        if (!found || t < best) {
+                            found = true;
+                            best = t;
+                            btri = p.tri;
+                            bu = u;
+                            bv = v;
+                            tie = 0;
+                        } else if (t == best && p.tri != btri) {
+                            tie = 1;
+                        }
+                    }
+                }
+            } else {
+                float4 n0, n1, n2, n3;
+                load_node(G, L, cur, n0, n1, n2, n3);
+                const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
+                const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
+                uint32_t lref = __float_as_uint(n0.w), rref = __float_as_uint(n1.w);
+                float tmax = found ? best : FLT_MAX;
+                float a = 0.0f, b = 0.0f;
+                if (COUNT) cnt.aabb += 2;
+                bool hl = ray_aabb(lmin, lmax, o, inv, tmax, a);
+                bool hr = ray_aabb(rmin, rmax, o, inv, tmax, b);
+                if (hl && hr) {
+                    bool lnear = a <= b;
+                    uint32_t far_ref = lnear ? rref : lref;
+                    float far_t = lnear ? b : a;
+                    if (sp < SK) {
+                        L.st_ref[sp * L.bs + L.lane] = far_ref;
+                        L.st_t[sp * L.bs + L.lane] = far_t;
+                    } else {
+                        ovf_ref[sp - SK] = far_ref;
+                        ovf_t[sp - SK] = far_t;
+                    }
+                    sp++;
+                    cur = lnear ? lref : rref;
+                    cur_t = lnear ? a : b;
+                    continue;
+                }
+                if (hl || hr) {
+                    cur = hl ? lref : rref;
+                    cur_t = hl ? a : b;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        if (sp < SK) {
+            cur = L.st_ref[sp * L.bs + L.lane];
+            cur_t = L.st_t[sp * L.bs + L.lane];
+        } else {
+            cur = ovf_ref[sp - SK];
+            cur_t = ovf_t[sp - SK];
+        }
+    }
+    if (!found) return false;
+    t_out = best;
+    tri_out = btri;
+    flags = ((bu < 1e-6f || bv < 1e-6f || bu + bv > 1.0f - 1e-6f) ? 1u : 0u) | (tie ? 2u : 0u);
+    return true;
+}
+
 // One crossing of a segment: position along it (0..1), original triangle,
 // authored surface and material.
 struct SegHit {

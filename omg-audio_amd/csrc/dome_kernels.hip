@@ -162,34 +162,60 @@ __global__ __launch_bounds__(128) void k_dome_rays(DomeArgs A)
 }
 
 // dome.rs:433-459 binning: e[bin][b] += tp[b] / n_rays and the
-// departure-direction sum, in fan order, one lane per (eye, bin).
+// departure-direction sum, in fan order.  One wave per eye: the wave loads
+// 64 rays at a time (coalesced) and computes each ray's terms in parallel;
+// lanes 0..8 then own one bin each and add the terms of the chunk's rays
+// in fan order, which keeps the reference's sequential f32 sums exact.
 template <int NBM>
-__global__ void k_dome_reduce(DomeArgs A)
+__global__ __launch_bounds__(64) void k_dome_reduce(DomeArgs A)
 {
-    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= A.n_eyes * kDomeBins) return;
-    uint32_t eye_i = gid / kDomeBins, k = gid - eye_i * kDomeBins;
+    uint32_t eye_i = blockIdx.x, lane = threadIdx.x;
     float e[NBM];
 #pragma unroll
     for (int b = 0; b < NBM; b++) e[b] = 0.0f;
     f3 ds = mk(0.0f, 0.0f, 0.0f);
     float nr = (float)A.n_rays;
     const DomeRayDev* rays = A.rays + (size_t)eye_i * A.n_rays;
-    for (uint32_t j = 0; j < A.n_rays; j++) {
-        const DomeRayDev& r = rays[j];
-        if (r.bin != k || !(r.tp[0] > 1e-9f)) continue;
+    for (uint32_t base = 0; base < A.n_rays; base += 64) {
+        uint32_t j = base + lane;
+        uint32_t bin = 0xFFFFFFFFu;
+        float c[NBM];
+        f3 dc = mk(0.0f, 0.0f, 0.0f);
 #pragma unroll
-        for (int b = 0; b < NBM; b++) e[b] += r.tp[b] / nr;
-        float w = fmaxf(r.tp[1 < NBM ? 1 : 0], r.tp[0] * 0.2f);
-        ds = add(ds, mul(mk(r.dir[0], r.dir[1], r.dir[2]), w));
+        for (int b = 0; b < NBM; b++) c[b] = 0.0f;
+        if (j < A.n_rays) {
+            const DomeRayDev& r = rays[j];
+            if (r.tp[0] > 1e-9f) {
+                bin = r.bin;
+#pragma unroll
+                for (int b = 0; b < NBM; b++) c[b] = r.tp[b] / nr;
+                float w = fmaxf(r.tp[1 < NBM ? 1 : 0], r.tp[0] * 0.2f);
+                dc = mul(mk(r.dir[0], r.dir[1], r.dir[2]), w);
+            }
+        }
+        uint32_t n = A.n_rays - base < 64u ? A.n_rays - base : 64u;
+        for (uint32_t l = 0; l < n; l++) {
+            uint32_t bl = __shfl(bin, (int)l);
+            float cl[NBM];
+#pragma unroll
+            for (int b = 0; b < NBM; b++) cl[b] = __shfl(c[b], (int)l);
+            f3 dl = mk(__shfl(dc.x, (int)l), __shfl(dc.y, (int)l), __shfl(dc.z, (int)l));
+            if (bl == lane) {
+#pragma unroll
+                for (int b = 0; b < NBM; b++) e[b] += cl[b];
+                ds = add(ds, dl);
+            }
+        }
     }
-    float* oe = A.energy + ((size_t)eye_i * kDomeBins + k) * NBM;
+    if (lane < kDomeBins) {
+        float* oe = A.energy + ((size_t)eye_i * kDomeBins + lane) * NBM;
 #pragma unroll
-    for (int b = 0; b < NBM; b++) oe[b] = e[b];
-    float* od = A.dsum + ((size_t)eye_i * kDomeBins + k) * 3;
-    od[0] = ds.x;
-    od[1] = ds.y;
-    od[2] = ds.z;
+        for (int b = 0; b < NBM; b++) oe[b] = e[b];
+        float* od = A.dsum + ((size_t)eye_i * kDomeBins + lane) * 3;
+        od[0] = ds.x;
+        od[1] = ds.y;
+        od[2] = ds.z;
+    }
 }
 
 template <int NBM>
@@ -202,8 +228,7 @@ static hipError_t launch_dome_t(const DomeArgs& A, bool count, hipStream_t s)
         else
             hipLaunchKernelGGL((k_dome_rays<NBM, false>), dim3((n + 127) / 128), dim3(128), 0, s, A);
     }
-    uint32_t m = A.n_eyes * kDomeBins;
-    if (m) hipLaunchKernelGGL((k_dome_reduce<NBM>), dim3((m + 63) / 64), dim3(64), 0, s, A);
+    if (A.n_eyes) hipLaunchKernelGGL((k_dome_reduce<NBM>), dim3(A.n_eyes), dim3(64), 0, s, A);
     return hipGetLastError();
 }
 

@@ -62,6 +62,9 @@ struct omg_ctx {
     uint32_t fan_n = 0;
     // scenes alive on this context; omg_shutdown defers the teardown until the last is destroyed
     uint32_t live_scenes = 0;
+    // tracer launch configuration (omg_set_launch_config)
+    uint32_t node_cache = 512;
+    uint32_t stack_lds = 16;
     bool closing = false;
 };
 

@@ -294,11 +294,22 @@ int build_scene(const float* positions, uint32_t nv, const uint32_t* indices, ui
     B.build(0, prims.size(), 1);
     S.max_depth = B.max_depth;
     S.n_bin_nodes = (uint32_t)B.nodes.size();
-    // flatten: internal nodes in preorder become Node2 entries
+    // flatten: internal nodes in breadth-first order become Node2 entries, so
+    // the first entries are the top levels every traversal crosses (the
+    // kernels keep a prefix of them in LDS)
     std::vector<int32_t> remap(B.nodes.size(), -1);
     uint32_t n_int = 0;
-    for (size_t i = 0; i < B.nodes.size(); i++)
-        if (B.nodes[i].left >= 0) remap[i] = (int32_t)n_int++;
+    {
+        std::vector<int32_t> queue;
+        queue.reserve(B.nodes.size());
+        if (B.nodes[0].left >= 0) queue.push_back(0);
+        for (size_t q = 0; q < queue.size(); q++) {
+            int32_t bi = queue[q];
+            remap[bi] = (int32_t)n_int++;
+            for (int32_t c : {B.nodes[bi].left, B.nodes[bi].right})
+                if (B.nodes[c].left >= 0) queue.push_back(c);
+        }
+    }
     auto ref_of = [&](int32_t bi) -> uint32_t {
         const BNode& n = B.nodes[bi];
         if (n.left < 0) return leaf_ref(n.start, n.count);

@@ -37,6 +37,8 @@ struct TraceArgs {
     unsigned long long* counters;  // [3] or null
     uint32_t ray_base;       // first global ray of this launch (chunked launches)
     uint32_t ray_count;      // rays in this launch
+    uint32_t node_cache;     // BVH nodes (top levels) staged in LDS per block
+    uint32_t stack_lds;      // traversal stack entries per lane kept in LDS (16 or 32)
 };
 
 }  // namespace omg

@@ -83,14 +83,15 @@ __device__ __forceinline__ void overlay_hits(const TraceArgs& A, f3 p, f3 d, boo
     }
 }
 
-template <bool MESH, bool COUNT>
-__device__ __forceinline__ bool raycast_hit(const TraceArgs& A, f3 p, f3 d, Hit& h, TravCount& c)
+template <bool MESH, bool COUNT, int SK>
+__device__ __forceinline__ bool raycast_hit(const TraceArgs& A, const LdsTrav& L, f3 p, f3 d,
+                                            Hit& h, TravCount& c)
 {
     bool found;
     if (MESH) {
         float t;
         uint32_t tri, fl;
-        found = mesh_raycast<COUNT>(A.g, p, d, t, tri, fl, c);
+        found = mesh_raycast_lds<COUNT, SK>(A.g, L, p, d, t, tri, fl, c);
         if (found) {
             float4 ti = *reinterpret_cast<const float4*>(A.g.tris + tri);
             f3 n = mk(ti.x, ti.y, ti.z);
@@ -109,10 +110,31 @@ __device__ __forceinline__ bool raycast_hit(const TraceArgs& A, f3 p, f3 d, Hit&
 }
 
 // tracer::trace (tracer.rs:80-202), one lane per (source, ray).
-template <int NBM, bool MESH, bool COUNT>
+//
+// Dynamic LDS: the first nc BVH nodes (breadth-first, so the top levels),
+// then the SK-deep stack tops of the block's lanes.  Blocks are dispatched
+// round-robin over the 8 XCDs; the block index is remapped so that each
+// XCD walks a contiguous range of rays (the same sources, hence the same
+// parts of the tree in its own L2).
+template <int NBM, bool MESH, bool COUNT, int SK>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 {
-    uint32_t lid = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ float4 lds[];
+    LdsTrav L;
+    L.nc = MESH ? A.node_cache : 0u;
+    L.nodes = lds;
+    L.st_ref = reinterpret_cast<uint32_t*>(lds + 4 * L.nc);
+    L.st_t = reinterpret_cast<float*>(L.st_ref + SK * blockDim.x);
+    L.lane = threadIdx.x;
+    L.bs = blockDim.x;
+    if (MESH) {
+        const float4* src4 = reinterpret_cast<const float4*>(A.g.nodes);
+        for (uint32_t i = threadIdx.x; i < 4 * L.nc; i += blockDim.x) lds[i] = src4[i];
+        __syncthreads();
+    }
+    uint32_t nb = gridDim.x, xcd = blockIdx.x & 7u, per = nb >> 3;
+    uint32_t blk = (nb & 7u) == 0 ? xcd * per + (blockIdx.x >> 3) : blockIdx.x;
+    uint32_t lid = blk * blockDim.x + threadIdx.x;
     bool active = lid < A.ray_count;
     uint64_t aabb_acc = 0, tri_acc = 0, iters_acc = 0;
     if (active) {
@@ -135,16 +157,16 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
         for (uint32_t bounce = 0; bounce < A.max_bounces; bounce++) {
             iters++;
             Hit h;
-            bool has = raycast_hit<MESH, COUNT>(A, pos, dir, h, tc);
+            bool has = raycast_hit<MESH, COUNT, SK>(A, L, pos, dir, h, tc);
             float t_hit = has ? h.t : FLT_MAX;
             // receiver sphere crossing along this segment (tracer.rs:110-128)
             for (uint32_t li = 0; li < A.n_lis; li++) {
-                f3 L = mk(A.listeners[3 * li], A.listeners[3 * li + 1], A.listeners[3 * li + 2]);
-                f3 to_l = sub(L, pos);
+                f3 Lp = mk(A.listeners[3 * li], A.listeners[3 * li + 1], A.listeners[3 * li + 2]);
+                f3 to_l = sub(Lp, pos);
                 float s = dot(to_l, dir);
                 if (s > 0.0f && s < t_hit) {
                     f3 closest = add(pos, mul(dir, s));
-                    if (length(sub(closest, L)) < A.radius) {
+                    if (length(sub(closest, Lp)) < A.radius) {
                         float arrival = (dist_total + s) / kSpeedOfSound;
                         uint32_t bin = sat_u32(arrival / A.bin_dt);
                         if (bin < A.nbins) {
@@ -240,27 +262,44 @@ __global__ void k_fill_zero_u64(unsigned long long* p, size_t n)
 
 // ------------------------------------------------------------------ launchers
 
-template <int NBM, bool MESH, bool COUNT>
+constexpr uint32_t kTraceBlock = 256;
+
+template <int NBM, bool MESH, bool COUNT, int SK>
 static hipError_t launch_trace_t(const TraceArgs& A, hipStream_t s)
 {
-    uint32_t blocks = (A.ray_count + 255) / 256;
+    uint32_t blocks = (A.ray_count + kTraceBlock - 1) / kTraceBlock;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_trace<NBM, MESH, COUNT>), dim3(blocks), dim3(256), 0, s, A);
+    size_t lds = (MESH ? (size_t)A.node_cache * sizeof(Node2) : 0) + (size_t)SK * kTraceBlock * 8;
+    auto kern = k_trace<NBM, MESH, COUNT, SK>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kTraceBlock), lds, s, A);
     return hipGetLastError();
+}
+
+template <int NBM, bool MESH, bool COUNT>
+static hipError_t launch_trace_sk(const TraceArgs& A, hipStream_t s)
+{
+    if (!MESH) return launch_trace_t<NBM, MESH, COUNT, 1>(A, s);
+    if (A.stack_lds >= 32) return launch_trace_t<NBM, MESH, COUNT, 32>(A, s);
+    return launch_trace_t<NBM, MESH, COUNT, 16>(A, s);
 }
 
 template <bool MESH, bool COUNT>
 static hipError_t launch_trace_nb(const TraceArgs& A, hipStream_t s)
 {
     switch (A.nbands) {
-    case 1: return launch_trace_t<1, MESH, COUNT>(A, s);
-    case 2: return launch_trace_t<2, MESH, COUNT>(A, s);
-    case 3: return launch_trace_t<3, MESH, COUNT>(A, s);
-    case 4: return launch_trace_t<4, MESH, COUNT>(A, s);
-    case 5: return launch_trace_t<5, MESH, COUNT>(A, s);
-    case 6: return launch_trace_t<6, MESH, COUNT>(A, s);
-    case 7: return launch_trace_t<7, MESH, COUNT>(A, s);
-    case 8: return launch_trace_t<8, MESH, COUNT>(A, s);
+    case 1: return launch_trace_sk<1, MESH, COUNT>(A, s);
+    case 2: return launch_trace_sk<2, MESH, COUNT>(A, s);
+    case 3: return launch_trace_sk<3, MESH, COUNT>(A, s);
+    case 4: return launch_trace_sk<4, MESH, COUNT>(A, s);
+    case 5: return launch_trace_sk<5, MESH, COUNT>(A, s);
+    case 6: return launch_trace_sk<6, MESH, COUNT>(A, s);
+    case 7: return launch_trace_sk<7, MESH, COUNT>(A, s);
+    case 8: return launch_trace_sk<8, MESH, COUNT>(A, s);
     default: return hipErrorInvalidValue;
     }
 }

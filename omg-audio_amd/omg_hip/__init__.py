@@ -89,6 +89,10 @@ class Context:
         check(lib().omg_device_name(self.h, buf, 128))
         self.device_name = buf.value.decode()
 
+    def set_launch_config(self, node_cache=512, stack_lds=16):
+        """Tracer launch shape: top BVH nodes staged in LDS, LDS stack depth."""
+        check(lib().omg_set_launch_config(self.h, node_cache, stack_lds))
+
     def close(self):
         if getattr(self, "h", None):
             lib().omg_shutdown(self.h)
