@@ -109,10 +109,11 @@ void omg_shutdown(omg_ctx *ctx);
 const char *omg_last_error(void);
 int omg_device_name(omg_ctx *ctx, char *buf, uint32_t len);
 /* Tracer launch shape (no reference counterpart; set by the autotuner):
- * BVH nodes of the top levels staged in LDS per block, and traversal-stack
- * entries per lane kept in LDS (16 or 32; deeper entries spill to scratch).
+ * BVH nodes of the top levels staged in LDS per block, traversal-stack
+ * entries per ray kept in LDS (16 or 32; deeper entries spill to scratch),
+ * and lanes per ray (1, 4 = a DPP quad per ray, 0 = by launch size).
  * Results do not depend on it. */
-int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds);
+int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds, uint32_t lanes);
 
 /* ------------------------------------------------------------ scenes */
 
@@ -163,6 +164,15 @@ int omg_trace_world(omg_scene *scene, const omg_trace_params *params, const floa
                     uint32_t n_src, const float *listeners, uint32_t n_lis,
                     const omg_overlay *overlays, uint32_t n_ovl, float *bins, float *dirs,
                     omg_ray_record *records, omg_counters *counters);
+
+/* omg_trace_world with the per-bounce hit record of every ray (parity
+ * checking; no reference counterpart): hit_ids and hit_t are
+ * [n_src * n_rays][max_bounces], the triangle id (0xFFFF0000 | i for
+ * overlay i, 0xFFFFFFFF = none) and distance of each bounce's hit. */
+int omg_trace_world_hits(omg_scene *scene, const omg_trace_params *params, const float *sources,
+                         uint32_t n_src, const float *listeners, uint32_t n_lis,
+                         const omg_overlay *overlays, uint32_t n_ovl, float *bins, float *dirs,
+                         omg_ray_record *records, uint32_t *hit_ids, float *hit_t);
 
 /* LateBackend::trace (late.rs:28-40; omg-gpu/src/lib.rs:1051-1067) over the
  * analytic Shoebox (scene.rs:25-82): size[3], walls[6] in the order

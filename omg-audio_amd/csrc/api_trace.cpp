@@ -72,15 +72,26 @@ extern "C" void omg_shutdown(omg_ctx* c)
     if (c->live_scenes == 0) ctx_teardown(c);
 }
 
-extern "C" int omg_set_launch_config(omg_ctx* c, uint32_t node_cache, uint32_t stack_lds)
+extern "C" int omg_set_launch_config(omg_ctx* c, uint32_t node_cache, uint32_t stack_lds,
+                                     uint32_t lanes)
 {
     if (!c) return fail(OMG_ERR_INVALID, "bad argument");
     if (stack_lds != 16 && stack_lds != 32) return fail(OMG_ERR_INVALID, "stack_lds must be 16 or 32");
+    if (lanes != 0 && lanes != 1 && lanes != 4) return fail(OMG_ERR_INVALID, "lanes must be 0, 1 or 4");
     size_t lds = (size_t)node_cache * sizeof(Node2) + (size_t)stack_lds * 256 * 8;
     if (lds > 160 * 1024) return fail(OMG_ERR_CAP, "LDS working set above 160 KiB");
     c->node_cache = node_cache;
     c->stack_lds = stack_lds;
+    c->lanes = lanes;
     return OMG_OK;
+}
+
+// Lanes per ray for a launch of n rays: a quad per ray while one lane per
+// ray would leave the 1024 SIMDs with few waves to switch between.
+uint32_t omg::pick_lanes(const omg_ctx* c, uint64_t n_rays)
+{
+    if (c->lanes) return c->lanes;
+    return n_rays < c->quad_below ? 4u : 1u;
 }
 
 extern "C" int omg_device_name(omg_ctx* c, char* buf, uint32_t len)
@@ -318,10 +329,12 @@ int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatPara
 
 int run_trace_host(omg_ctx* c, TraceArgs& A, bool mesh, uint32_t nbins, uint32_t nbands,
                    const float* sources, uint32_t n_src, const float* listeners, uint32_t n_lis,
-                   float* bins, float* dirs, omg_ray_record* records, omg_counters* counters)
+                   float* bins, float* dirs, omg_ray_record* records, omg_counters* counters,
+                   uint32_t* hit_ids = nullptr, float* hit_t = nullptr)
 {
     size_t total = (size_t)A.n_rays * n_src;
     size_t nb_f = (size_t)n_src * n_lis * nbins * nbands, nd_f = (size_t)n_src * n_lis * nbins * 3;
+    size_t nh = total * A.max_bounces;
     int rc = upload(c->job_src, sources, (size_t)n_src * 3 * sizeof(float), c->stream);
     if (!rc) rc = upload(c->job_lis, listeners, (size_t)n_lis * 3 * sizeof(float), c->stream);
     if (rc) return rc;
@@ -329,6 +342,15 @@ int run_trace_host(omg_ctx* c, TraceArgs& A, bool mesh, uint32_t nbins, uint32_t
     OMG_HIP_CHECK(c->job_dirs.ensure(nd_f * sizeof(float)));
     if (records) OMG_HIP_CHECK(c->job_rec.ensure(total * sizeof(omg_ray_record)));
     if (counters) OMG_HIP_CHECK(c->job_cnt.ensure(3 * sizeof(unsigned long long)));
+    if (hit_ids) {
+        OMG_HIP_CHECK(c->aux0.ensure(nh * sizeof(uint32_t)));
+        OMG_HIP_CHECK(c->aux1.ensure(nh * sizeof(float)));
+        A.hit_ids = c->aux0.as<uint32_t>();
+        A.hit_t = c->aux1.as<float>();
+        // rays that end early leave their later slots as "no hit"
+        OMG_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)A.hit_ids, 0xFFFFFFFFu, nh, c->stream));
+        OMG_HIP_CHECK(launch_fill_zero(A.hit_t, nh, c->stream));
+    }
     A.sources = c->job_src.as<float>();
     A.listeners = c->job_lis.as<float>();
     A.n_src = n_src;
@@ -352,6 +374,10 @@ int run_trace_host(omg_ctx* c, TraceArgs& A, bool mesh, uint32_t nbins, uint32_t
     unsigned long long cnt[3] = {0, 0, 0};
     if (counters)
         OMG_HIP_CHECK(hipMemcpyAsync(cnt, A.counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (hit_ids) {
+        OMG_HIP_CHECK(hipMemcpyAsync(hit_ids, A.hit_ids, nh * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        OMG_HIP_CHECK(hipMemcpyAsync(hit_t, A.hit_t, nh * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
     OMG_HIP_CHECK(hipStreamSynchronize(c->stream));
     if (counters) {
         counters->aabb_tests = cnt[0];
@@ -394,10 +420,38 @@ extern "C" int omg_trace_world(omg_scene* s, const omg_trace_params* p, const fl
     fill_geom(s, A.g);
     A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
     A.stack_lds = c->stack_lds;
+    A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
                           dirs, records, counters);
+}
+
+extern "C" int omg_trace_world_hits(omg_scene* s, const omg_trace_params* p, const float* sources,
+                                    uint32_t n_src, const float* listeners, uint32_t n_lis,
+                                    const omg_overlay* overlays, uint32_t n_ovl, float* bins,
+                                    float* dirs, omg_ray_record* records, uint32_t* hit_ids,
+                                    float* hit_t)
+{
+    if (!s || !sources || !listeners || !bins || !dirs || !records || !hit_ids || !hit_t ||
+        (n_ovl && !overlays))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    uint32_t nbins;
+    int rc = validate_params(p, s->nbands, n_src, n_lis, n_ovl, &nbins);
+    if (rc) return rc;
+    if ((uint64_t)p->n_rays * n_src * p->max_bounces > (1ull << 28))
+        return fail(OMG_ERR_CAP, "per-bounce hit record too large");
+    omg_ctx* c = s->ctx;
+    OMG_HIP_CHECK(hipSetDevice(c->device));
+    TraceArgs A{};
+    fill_geom(s, A.g);
+    A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
+    A.stack_lds = c->stack_lds;
+    A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
+    rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
+    if (rc) return rc;
+    return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
+                          dirs, records, nullptr, hit_ids, hit_t);
 }
 
 extern "C" int omg_trace_box(omg_ctx* c, const float size[3], const omg_material walls[6],
@@ -446,6 +500,7 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
     fill_geom(s, A.g);
     A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
     A.stack_lds = c->stack_lds;
+    A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
     // the job tables are staged on the context stream, then ordered before the launch
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;

@@ -75,6 +75,41 @@ struct TravCount {
     uint32_t aabb = 0, tri = 0;
 };
 
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+
+// Nearest-hit state of one raycast.  "No hit yet" is btri == kNoHit, an
+// integer kept in a VGPR like the rest of the state: a bool here becomes a
+// lane mask in SGPRs, which the compiler failed to reset for every lane
+// between two raycasts of the quad traversal.
+struct BestHit {
+    float best = FLT_MAX, bu = 0.0f, bv = 0.0f;
+    uint32_t btri = kNoHit, tie = 0;
+
+    __device__ __forceinline__ bool found() const { return btri != kNoHit; }
+    // strict `t < best`; an exact-t tie with another triangle is flagged
+    __device__ __forceinline__ void offer(float t, uint32_t tri, float u, float v)
+    {
+        if (!found() || t < best) {
+            best = t;
+            btri = tri;
+            bu = u;
+            bv = v;
+            tie = 0;
+        } else if (t == best && tri != btri) {
+            tie = 1;
+        }
+    }
+    // flags: bit0 winning hit within 1e-6 barycentric of an edge, bit1 exact-t tie
+    __device__ __forceinline__ bool result(float& t_out, uint32_t& tri_out, uint32_t& flags) const
+    {
+        if (!found()) return false;
+        t_out = best;
+        tri_out = btri;
+        flags = ((bu < 1e-6f || bv < 1e-6f || bu + bv > 1.0f - 1e-6f) ? 1u : 0u) | (tie ? 2u : 0u);
+        return true;
+    }
+};
+
 // Mesh::raycast (mesh.rs:484-541): ordered traversal, near child popped
 // first (ties to the left child), strict `t < best`, pop-time cull on
 // `t_enter >= best`.  flags: bit0 winning hit within 1e-6 barycentric of
@@ -90,14 +125,12 @@ __device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, floa
     int sp = 1;
     st_ref[0] = G.root_ref;
     st_t[0] = 0.0f;
-    bool found = false;
-    float best = FLT_MAX, bu = 0.0f, bv = 0.0f;
-    uint32_t btri = 0, tie = 0;
+    BestHit H;
     while (sp > 0) {
         sp--;
         uint32_t ref = st_ref[sp];
         float te = st_t[sp];
-        if (found && te >= best) continue;
+        if (H.found() && te >= H.best) continue;
         if (ref & kLeafBit) {
             uint32_t start = ref & kLeafStartMask;
             uint32_t cnt_p = (ref >> kLeafCountShift) & 7u;
@@ -105,18 +138,7 @@ __device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, floa
                 Prim p = load_prim(G.prims, k);
                 if (COUNT) cnt.tri++;
                 float t, u, v;
-                if (ray_tri(p, o, d, t, u, v)) {
-                    if (!found || t < best) {
-                        found = true;
-                        best = t;
-                        btri = p.tri;
-                        bu = u;
-                        bv = v;
-                        tie = 0;
-                    } else if (t == best && p.tri != btri) {
-                        tie = 1;
-                    }
-                }
+                if (ray_tri(p, o, d, t, u, v)) H.offer(t, p.tri, u, v);
             }
         } else {
             const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
@@ -124,7 +146,7 @@ __device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, floa
             const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
             const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
             uint32_t lref = __float_as_uint(n0.w), rref = __float_as_uint(n1.w);
-            float tmax = found ? best : FLT_MAX;
+            float tmax = H.found() ? H.best : FLT_MAX;
             float a = 0.0f, b = 0.0f;
             if (COUNT) cnt.aabb += 2;
             bool hl = ray_aabb(lmin, lmax, o, inv, tmax, a);
@@ -147,23 +169,36 @@ __device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, floa
             }
         }
     }
-    if (!found) return false;
-    t_out = best;
-    tri_out = btri;
-    flags = ((bu < 1e-6f || bv < 1e-6f || bu + bv > 1.0f - 1e-6f) ? 1u : 0u) | (tie ? 2u : 0u);
+    return H.result(t_out, tri_out, flags);
     return true;
 }
 
+
+// Pointers into LDS carry address space 3 explicitly.  With generic
+// pointers the compiler may merge an LDS access and a scratch/global access
+// of the two sides of a branch into one FLAT access through a selected
+// pointer; a FLAT read of LDS is not ordered after an earlier DS write of
+// the same wave, so a stack entry pushed with ds_write could be popped stale.
+#define OMG_LDS __attribute__((address_space(3)))
+typedef OMG_LDS float4 lds_float4;
+typedef OMG_LDS uint32_t lds_u32;
+typedef OMG_LDS float lds_f32;
+
+__device__ __forceinline__ float4 lds_ld4(const lds_float4* p)
+{
+    const lds_f32* f = reinterpret_cast<const lds_f32*>(p);
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
 
 // LDS working set of one block for mesh_raycast_lds: a prefix of the node
 // array (the top levels, nodes are stored breadth-first) and the top SK
 // entries of every lane's traversal stack, laid out [depth][lane] so that
 // lanes at the same depth hit distinct banks.
 struct LdsTrav {
-    const float4* nodes;   // [nc][4]
+    const lds_float4* nodes;   // [nc][4]
     uint32_t nc;
-    uint32_t* st_ref;      // [SK][bs]
-    float* st_t;           // [SK][bs]
+    lds_u32* st_ref;           // [SK][bs]
+    lds_f32* st_t;             // [SK][bs]
     uint32_t lane, bs;
 };
 
@@ -171,8 +206,11 @@ __device__ __forceinline__ void load_node(const GeomArgs& G, const LdsTrav& L, u
                                           float4& n0, float4& n1, float4& n2, float4& n3)
 {
     if (ref < L.nc) {
-        const float4* q = L.nodes + 4 * ref;
-        n0 = q[0]; n1 = q[1]; n2 = q[2]; n3 = q[3];
+        const lds_float4* q = L.nodes + 4 * ref;
+        n0 = lds_ld4(q);
+        n1 = lds_ld4(q + 1);
+        n2 = lds_ld4(q + 2);
+        n3 = lds_ld4(q + 3);
     } else {
         const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
         n0 = q[0]; n1 = q[1]; n2 = q[2]; n3 = q[3];
@@ -195,11 +233,9 @@ __device__ __forceinline__ bool mesh_raycast_lds(const GeomArgs& G, const LdsTra
     int sp = 0;
     uint32_t cur = G.root_ref;
     float cur_t = 0.0f;
-    bool found = false;
-    float best = FLT_MAX, bu = 0.0f, bv = 0.0f;
-    uint32_t btri = 0, tie = 0;
+    BestHit H;
     while (true) {
-        if (!(found && cur_t >= best)) {
+        if (!(H.found() && cur_t >= H.best)) {
             if (cur & kLeafBit) {
                 uint32_t start = cur & kLeafStartMask;
                 uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
@@ -207,18 +243,7 @@ __device__ __forceinline__ bool mesh_raycast_lds(const GeomArgs& G, const LdsTra
                     Prim p = load_prim(G.prims, k);
                     if (COUNT) cnt.tri++;
                     float t, u, v;
-                    if (ray_tri(p, o, d, t, u, v)) {
-                        if (!found || t < best) {
-                            found = true;
-                            best = t;
-                            btri = p.tri;
-                            bu = u;
-                            bv = v;
-                            tie = 0;
-                        } else if (t == best && p.tri != btri) {
-                            tie = 1;
-                        }
-                    }
+                    if (ray_tri(p, o, d, t, u, v)) H.offer(t, p.tri, u, v);
                 }
             } else {
                 float4 n0, n1, n2, n3;
@@ -226,7 +251,7 @@ __device__ __forceinline__ bool mesh_raycast_lds(const GeomArgs& G, const LdsTra
                 const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
                 const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
                 uint32_t lref = __float_as_uint(n0.w), rref = __float_as_uint(n1.w);
-                float tmax = found ? best : FLT_MAX;
+                float tmax = H.found() ? H.best : FLT_MAX;
                 float a = 0.0f, b = 0.0f;
                 if (COUNT) cnt.aabb += 2;
                 bool hl = ray_aabb(lmin, lmax, o, inv, tmax, a);
@@ -264,11 +289,136 @@ __device__ __forceinline__ bool mesh_raycast_lds(const GeomArgs& G, const LdsTra
             cur_t = ovf_t[sp - SK];
         }
     }
-    if (!found) return false;
-    t_out = best;
-    tri_out = btri;
-    flags = ((bu < 1e-6f || bv < 1e-6f || bu + bv > 1.0f - 1e-6f) ? 1u : 0u) | (tie ? 2u : 0u);
-    return true;
+    return H.result(t_out, tri_out, flags);
+}
+
+// ------------------------------------------------------------------ quad traversal
+//
+// Four lanes (a DPP quad) trace one ray together.  Every lane of the quad
+// holds the same traversal state; the work of one step is split: at an
+// internal node lane 0 tests the left child box and lane 1 the right one, at
+// a leaf lane q tests primitive q.  Results are broadcast inside the quad
+// with DPP quad_perm moves and folded in the reference's order (left/right,
+// then primitives in leaf order), so the outcome is the one-lane traversal's.
+
+template <int Q>
+__device__ __forceinline__ uint32_t qb(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, Q | (Q << 2) | (Q << 4) | (Q << 6), 0xF, 0xF,
+                                              false);
+}
+template <int Q>
+__device__ __forceinline__ float qbf(float x) { return __uint_as_float(qb<Q>(__float_as_uint(x))); }
+
+// LDS stack of one quad: written by lane 0, read by all four (same address,
+// a broadcast read).  Laid out [depth][quad].
+struct QuadTrav {
+    const lds_float4* nodes;   // LDS node prefix [nc][4]
+    uint32_t nc;
+    lds_u32* st_ref;           // [SK][quads per block]
+    lds_f32* st_t;
+    uint32_t quad, nq, q;      // quad index in the block, quads per block, lane in quad
+};
+
+template <int J>
+__device__ __forceinline__ void fold_prim(uint32_t cnt_p, uint32_t hit_q, float t_q, uint32_t tri_q,
+                                          float u_q, float v_q, BestHit& H)
+{
+    uint32_t h = qb<J>(hit_q);
+    float t = qbf<J>(t_q);
+    uint32_t tri = qb<J>(tri_q);
+    float u = qbf<J>(u_q), v = qbf<J>(v_q);
+    if ((uint32_t)J < cnt_p && h) H.offer(t, tri, u, v);
+}
+
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadTrav& L, f3 o, f3 d,
+                                                  float& t_out, uint32_t& tri_out, uint32_t& flags,
+                                                  TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    uint32_t ovf_ref[kStackCap - SK];
+    float ovf_t[kStackCap - SK];
+    int sp = 0;
+    uint32_t cur = G.root_ref;
+    float cur_t = 0.0f;
+    BestHit H;
+    const uint32_t q = L.q;
+    while (true) {
+        if (!(H.found() && cur_t >= H.best)) {
+            if (cur & kLeafBit) {
+                uint32_t start = cur & kLeafStartMask;
+                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+                uint32_t hit_q = 0, tri_q = 0;
+                float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+                if (q < cnt_p) {
+                    Prim p = load_prim(G.prims, start + q);
+                    tri_q = p.tri;
+                    hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+                }
+                if (COUNT && q == 0) cnt.tri += cnt_p;
+                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+            } else {
+                // lane 0: left child (q[0], q[1]); lane 1: right child (q[2], q[3])
+                uint32_t half = q & 1u;
+                float4 lo, hi;
+                if (cur < L.nc) {
+                    const lds_float4* src = L.nodes + 4 * cur;
+                    lo = lds_ld4(src + 2 * half);
+                    hi = lds_ld4(src + 2 * half + 1);
+                } else {
+                    const float4* src = reinterpret_cast<const float4*>(G.nodes + cur);
+                    lo = src[2 * half];
+                    hi = src[2 * half + 1];
+                }
+                const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
+                float tmax = H.found() ? H.best : FLT_MAX;
+                float te = 0.0f;
+                uint32_t hit = ray_aabb(bmin, bmax, o, inv, tmax, te) ? 1u : 0u;
+                uint32_t lref = qb<0>(__float_as_uint(lo.w)), rref = qb<0>(__float_as_uint(hi.w));
+                uint32_t hl = qb<0>(hit), hr = qb<1>(hit);
+                float a = qbf<0>(te), b = qbf<1>(te);
+                if (COUNT && q == 0) cnt.aabb += 2;
+                if (hl && hr) {
+                    bool lnear = a <= b;
+                    uint32_t far_ref = lnear ? rref : lref;
+                    float far_t = lnear ? b : a;
+                    if (sp < SK) {
+                        if (q == 0) {
+                            L.st_ref[sp * L.nq + L.quad] = far_ref;
+                            L.st_t[sp * L.nq + L.quad] = far_t;
+                        }
+                    } else {
+                        ovf_ref[sp - SK] = far_ref;
+                        ovf_t[sp - SK] = far_t;
+                    }
+                    sp++;
+                    cur = lnear ? lref : rref;
+                    cur_t = lnear ? a : b;
+                    continue;
+                }
+                if (hl || hr) {
+                    cur = hl ? lref : rref;
+                    cur_t = hl ? a : b;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        if (sp < SK) {
+            cur = L.st_ref[sp * L.nq + L.quad];
+            cur_t = L.st_t[sp * L.nq + L.quad];
+        } else {
+            cur = ovf_ref[sp - SK];
+            cur_t = ovf_t[sp - SK];
+        }
+    }
+    return H.result(t_out, tri_out, flags);
 }
 
 // One crossing of a segment: position along it (0..1), original triangle,

@@ -65,6 +65,8 @@ struct omg_ctx {
     // tracer launch configuration (omg_set_launch_config)
     uint32_t node_cache = 512;
     uint32_t stack_lds = 16;
+    uint32_t lanes = 0;                 // 0: pick per launch (pick_lanes)
+    uint64_t quad_below = 0;            // launches with fewer rays trace a quad per ray
     bool closing = false;
 };
 
@@ -90,5 +92,6 @@ hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s);
 hipError_t launch_fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s);
+uint32_t pick_lanes(const omg_ctx* c, uint64_t n_rays);
 void fill_geom(const omg_scene* s, GeomArgs& g);
 }  // namespace omg

@@ -38,7 +38,10 @@ struct TraceArgs {
     uint32_t ray_base;       // first global ray of this launch (chunked launches)
     uint32_t ray_count;      // rays in this launch
     uint32_t node_cache;     // BVH nodes (top levels) staged in LDS per block
-    uint32_t stack_lds;      // traversal stack entries per lane kept in LDS (16 or 32)
+    uint32_t stack_lds;      // traversal stack entries per ray kept in LDS (16 or 32)
+    uint32_t lanes;          // lanes per ray: 1, or 4 (a DPP quad)
+    uint32_t* hit_ids;       // [rays][max_bounces] hit id per bounce (0xFFFFFFFF none), or null
+    float* hit_t;            // [rays][max_bounces] hit distance per bounce, or null
 };
 
 }  // namespace omg

@@ -83,15 +83,30 @@ __device__ __forceinline__ void overlay_hits(const TraceArgs& A, f3 p, f3 d, boo
     }
 }
 
-template <bool MESH, bool COUNT, int SK>
-__device__ __forceinline__ bool raycast_hit(const TraceArgs& A, const LdsTrav& L, f3 p, f3 d,
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const LdsTrav& L, f3 o, f3 d,
+                                                 float& t, uint32_t& tri, uint32_t& fl,
+                                                 TravCount& c)
+{
+    return mesh_raycast_lds<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+}
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const QuadTrav& L, f3 o, f3 d,
+                                                 float& t, uint32_t& tri, uint32_t& fl,
+                                                 TravCount& c)
+{
+    return mesh_raycast_quad<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+}
+
+template <bool MESH, bool COUNT, int SK, class Trav>
+__device__ __forceinline__ bool raycast_hit(const TraceArgs& A, const Trav& L, f3 p, f3 d,
                                             Hit& h, TravCount& c)
 {
     bool found;
     if (MESH) {
         float t;
         uint32_t tri, fl;
-        found = mesh_raycast_lds<COUNT, SK>(A.g, L, p, d, t, tri, fl, c);
+        found = mesh_raycast_any<COUNT, SK>(A.g, L, p, d, t, tri, fl, c);
         if (found) {
             float4 ti = *reinterpret_cast<const float4*>(A.g.tris + tri);
             f3 n = mk(ti.x, ti.y, ti.z);
@@ -109,32 +124,59 @@ __device__ __forceinline__ bool raycast_hit(const TraceArgs& A, const LdsTrav& L
     return found;
 }
 
-// tracer::trace (tracer.rs:80-202), one lane per (source, ray).
+template <int LANES>
+struct TravOf {
+    using type = LdsTrav;
+};
+template <>
+struct TravOf<4> {
+    using type = QuadTrav;
+};
+
+__device__ __forceinline__ void init_trav(LdsTrav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    L.nc = nc;
+    L.nodes = lds;
+    L.st_ref = reinterpret_cast<lds_u32*>(lds + 4 * nc);
+    L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * blockDim.x);
+    L.lane = threadIdx.x;
+    L.bs = blockDim.x;
+}
+__device__ __forceinline__ void init_trav(QuadTrav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    L.nc = nc;
+    L.nodes = lds;
+    L.nq = blockDim.x >> 2;
+    L.st_ref = reinterpret_cast<lds_u32*>(lds + 4 * nc);
+    L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nq);
+    L.quad = threadIdx.x >> 2;
+    L.q = threadIdx.x & 3u;
+}
+
+// tracer::trace (tracer.rs:80-202).  LANES = 1: one lane per (source, ray);
+// LANES = 4: a DPP quad per ray (see mesh_raycast_quad), the quad's lanes
+// also splitting the listeners between them.
 //
 // Dynamic LDS: the first nc BVH nodes (breadth-first, so the top levels),
-// then the SK-deep stack tops of the block's lanes.  Blocks are dispatched
+// then the SK-deep stack tops of the block's rays.  Blocks are dispatched
 // round-robin over the 8 XCDs; the block index is remapped so that each
 // XCD walks a contiguous range of rays (the same sources, hence the same
 // parts of the tree in its own L2).
-template <int NBM, bool MESH, bool COUNT, int SK>
+template <int NBM, bool MESH, bool COUNT, int SK, int LANES>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 {
     extern __shared__ float4 lds[];
-    LdsTrav L;
-    L.nc = MESH ? A.node_cache : 0u;
-    L.nodes = lds;
-    L.st_ref = reinterpret_cast<uint32_t*>(lds + 4 * L.nc);
-    L.st_t = reinterpret_cast<float*>(L.st_ref + SK * blockDim.x);
-    L.lane = threadIdx.x;
-    L.bs = blockDim.x;
+    typename TravOf<LANES>::type L;
+    init_trav(L, (lds_float4*)lds, MESH ? A.node_cache : 0u, SK);
     if (MESH) {
         const float4* src4 = reinterpret_cast<const float4*>(A.g.nodes);
-        for (uint32_t i = threadIdx.x; i < 4 * L.nc; i += blockDim.x) lds[i] = src4[i];
+        for (uint32_t i = threadIdx.x; i < 4 * A.node_cache; i += blockDim.x) lds[i] = src4[i];
         __syncthreads();
     }
     uint32_t nb = gridDim.x, xcd = blockIdx.x & 7u, per = nb >> 3;
     uint32_t blk = (nb & 7u) == 0 ? xcd * per + (blockIdx.x >> 3) : blockIdx.x;
-    uint32_t lid = blk * blockDim.x + threadIdx.x;
+    uint32_t lid = (blk * blockDim.x + threadIdx.x) / LANES;
+    const uint32_t q = LANES == 1 ? 0u : (threadIdx.x & (LANES - 1));
     bool active = lid < A.ray_count;
     uint64_t aabb_acc = 0, tri_acc = 0, iters_acc = 0;
     if (active) {
@@ -160,7 +202,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
             bool has = raycast_hit<MESH, COUNT, SK>(A, L, pos, dir, h, tc);
             float t_hit = has ? h.t : FLT_MAX;
             // receiver sphere crossing along this segment (tracer.rs:110-128)
-            for (uint32_t li = 0; li < A.n_lis; li++) {
+            for (uint32_t li = q; li < A.n_lis; li += LANES) {
                 f3 Lp = mk(A.listeners[3 * li], A.listeners[3 * li + 1], A.listeners[3 * li + 2]);
                 f3 to_l = sub(Lp, pos);
                 float s = dot(to_l, dir);
@@ -184,6 +226,11 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
                         }
                     }
                 }
+            }
+            if (A.hit_ids && q == 0) {
+                size_t slot = (size_t)gid * A.max_bounces + bounce;
+                A.hit_ids[slot] = has ? h.id : 0xFFFFFFFFu;
+                A.hit_t[slot] = has ? h.t : 0.0f;
             }
             if (!has) { term = 0u; break; }
             hash = fnv1a_u32(hash, h.id);
@@ -221,11 +268,11 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
             }
             pos = add(pos, mul(n, 1e-3f));
         }
-        if (A.records) {
+        if (A.records && q == 0) {
             uint4 r = make_uint4(iters, term, hash, rflags);
             reinterpret_cast<uint4*>(A.records)[gid] = r;
         }
-        if (COUNT) {
+        if (COUNT && q == 0) {
             aabb_acc = tc.aabb;
             tri_acc = tc.tri;
             iters_acc = iters;
@@ -264,13 +311,15 @@ __global__ void k_fill_zero_u64(unsigned long long* p, size_t n)
 
 constexpr uint32_t kTraceBlock = 256;
 
-template <int NBM, bool MESH, bool COUNT, int SK>
-static hipError_t launch_trace_t(const TraceArgs& A, hipStream_t s)
+template <int NBM, bool MESH, bool COUNT, int SK, int LANES>
+static hipError_t launch_trace_l(const TraceArgs& A, hipStream_t s)
 {
-    uint32_t blocks = (A.ray_count + kTraceBlock - 1) / kTraceBlock;
+    uint64_t threads = (uint64_t)A.ray_count * LANES;
+    uint32_t blocks = (uint32_t)((threads + kTraceBlock - 1) / kTraceBlock);
     if (blocks == 0) return hipSuccess;
-    size_t lds = (MESH ? (size_t)A.node_cache * sizeof(Node2) : 0) + (size_t)SK * kTraceBlock * 8;
-    auto kern = k_trace<NBM, MESH, COUNT, SK>;
+    size_t lds = (MESH ? (size_t)A.node_cache * sizeof(Node2) : 0) +
+                 (size_t)SK * (kTraceBlock / LANES) * 8;
+    auto kern = k_trace<NBM, MESH, COUNT, SK, LANES>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -278,6 +327,13 @@ static hipError_t launch_trace_t(const TraceArgs& A, hipStream_t s)
     }
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kTraceBlock), lds, s, A);
     return hipGetLastError();
+}
+
+template <int NBM, bool MESH, bool COUNT, int SK>
+static hipError_t launch_trace_t(const TraceArgs& A, hipStream_t s)
+{
+    if (MESH && A.lanes == 4) return launch_trace_l<NBM, MESH, COUNT, SK, 4>(A, s);
+    return launch_trace_l<NBM, MESH, COUNT, SK, 1>(A, s);
 }
 
 template <int NBM, bool MESH, bool COUNT>

@@ -89,9 +89,10 @@ class Context:
         check(lib().omg_device_name(self.h, buf, 128))
         self.device_name = buf.value.decode()
 
-    def set_launch_config(self, node_cache=512, stack_lds=16):
-        """Tracer launch shape: top BVH nodes staged in LDS, LDS stack depth."""
-        check(lib().omg_set_launch_config(self.h, node_cache, stack_lds))
+    def set_launch_config(self, node_cache=512, stack_lds=16, lanes=0):
+        """Tracer launch shape: top BVH nodes staged in LDS, LDS stack depth,
+        lanes per ray (1, 4, or 0 = chosen by launch size)."""
+        check(lib().omg_set_launch_config(self.h, node_cache, stack_lds, lanes))
 
     def close(self):
         if getattr(self, "h", None):
@@ -226,6 +227,21 @@ def trace_world(scene, params, sources, listeners, overlays=(), records=False, c
     c = dict(aabb_tests=cnt.aabb_tests, tri_tests=cnt.tri_tests,
              bounce_rays=cnt.bounce_rays) if counters else {}
     return TraceResult(bins, dirs, rec, c)
+
+
+def trace_world_hits(scene, params, sources, listeners, overlays=()):
+    """omg_trace_world_hits: also returns every ray's per-bounce hit ids and distances."""
+    src = _f32(sources).reshape(-1, 3)
+    lis = _f32(listeners).reshape(-1, 3)
+    bins, dirs, rec = _alloc_out(params, src.shape[0], lis.shape[0], True)
+    n = src.shape[0] * params.n_rays
+    ids = np.zeros((n, params.max_bounces), np.uint32)
+    ts = np.zeros((n, params.max_bounces), np.float32)
+    ov = _overlays(list(overlays))
+    check(lib().omg_trace_world_hits(scene.h, C.byref(params), _p(src), src.shape[0], _p(lis),
+                                     lis.shape[0], C.cast(ov, C.c_void_p), len(overlays),
+                                     _p(bins), _p(dirs), _p(rec), _p(ids), _p(ts)))
+    return TraceResult(bins, dirs, rec), ids, ts
 
 
 def trace_box(ctx, size, walls, params, sources, listeners, records=False, counters=False):

@@ -103,7 +103,7 @@ SIGNATURES = [
     ("omg_shutdown", None, [_vp]),
     ("omg_last_error", C.c_char_p, []),
     ("omg_device_name", _i, [_vp, C.c_char_p, _u32]),
-    ("omg_set_launch_config", _i, [_vp, _u32, _u32]),
+    ("omg_set_launch_config", _i, [_vp, _u32, _u32, _u32]),
     ("omg_scene_create", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32,
                               C.POINTER(_vp)]),
     ("omg_scene_create_soup", _i, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32,
@@ -115,6 +115,8 @@ SIGNATURES = [
                                 C.POINTER(_u32)]),
     ("omg_scene_destroy", None, [_vp]),
     ("omg_trace_world", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("omg_trace_world_hits", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp,
+                                  _vp]),
     ("omg_trace_box", _i, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("omg_trace_world_async", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp,
                                    _vp]),

@@ -113,6 +113,31 @@ def test_city_block_multi_listener_parity(ctx, oracle):
     compare(gpu, outs, 1024, label="city block")
 
 
+@pytest.mark.parametrize("shape", [(0, 16, 1), (512, 32, 1), (0, 16, 4), (2048, 16, 4),
+                                   (512, 32, 4)])
+def test_launch_shapes_give_identical_rays(ctx, oracle, shape):
+    """Every launch shape (LDS node prefix, LDS stack depth, lanes per ray)
+    reproduces the oracle ray for ray: results do not depend on tuning."""
+    nc, sk, lanes = shape
+    mb, bounds, boxes = scenes.city_block(seed=5, n_buildings=8, target_tris=6000)
+    src = np.vstack([scenes.indoor_points(51, 2, boxes), scenes.outdoor_points(52, 1, bounds, boxes)])
+    lis = np.vstack([scenes.indoor_points(53, 3, boxes), scenes.outdoor_points(54, 2, bounds, boxes)])
+    _say(f"case launch shape node_cache={nc} stack_lds={sk} lanes={lanes}")
+    kw = dict(n_rays=1000, nbands=3, seed=31, bin_dt=0.001, max_time=2.0)
+    scene = H.Scene.from_builder(ctx, mb)
+    try:
+        ctx.set_launch_config(nc, sk, lanes)
+        gpu = H.trace_world(scene, H.trace_params(**kw), src, lis, records=True, counters=True)
+    finally:
+        ctx.set_launch_config()
+    om = oracle_mesh_from_builder(mb)
+    outs = oracle_trace_sources(O.mesh_geom(om), src, lis, **kw)
+    compare(gpu, outs, 1000, label=f"shape {shape}")
+    assert gpu.counters["aabb_tests"] == sum(int(o["stats"][0]) for o in outs)
+    assert gpu.counters["tri_tests"] == sum(int(o["stats"][1]) for o in outs)
+    assert gpu.counters["bounce_rays"] == sum(int(o["stats"][2]) for o in outs)
+
+
 def test_overlays_six_bands(ctx, oracle):
     mb, src, lis, _ = scenes.shoebox(seed=3, nbands=6)
     panel_mat = H.MaterialDef([0.1, 0.2, 0.3, 0.4, 0.5, 0.6], 0.2,
