@@ -192,6 +192,71 @@ int omg_trace_world_async(omg_scene *scene, const omg_trace_params *params,
                           float *d_bins, float *d_dirs, omg_ray_record *d_records,
                           void *stream);
 
+/* ------------------------------------------------------------ batched persistent launch */
+
+/* One query of a batch: the arguments of one omg_trace_world call
+ * (late.rs:81-91), host memory.  No reference counterpart for the batch
+ * itself: the reference runs one job per call. */
+typedef struct {
+    omg_scene *scene;
+    omg_trace_params params;
+    const float *sources;
+    uint32_t n_src;
+    const float *listeners;
+    uint32_t n_lis;
+    const omg_overlay *overlays;
+    uint32_t n_ovl;
+} omg_query;
+
+/* Launch shape of the persistent kernel: lanes per ray (1, or 4 = a DPP
+ * quad), traversal-stack entries per ray kept in LDS (16 or 32), resident
+ * waves per SIMD the grid is sized for (1..8), and regen = 1 to refill
+ * lanes whose ray has ended with new rays between bounces (lanes = 1
+ * only).  ms: the sampled time the autotuner measured for it (0 when set
+ * by hand). */
+typedef struct {
+    uint32_t lanes;
+    uint32_t stack_lds;
+    uint32_t waves_per_simd;
+    uint32_t regen;
+    float ms;
+} omg_batch_shape;
+
+#define OMG_BATCH_RECORDS 1u   /* keep per-ray records (omg_batch_result) */
+#define OMG_BATCH_COUNTERS 2u  /* count traversal work (omg_batch_counters) */
+
+typedef struct omg_batch omg_batch;
+
+/* Uploads every query's inputs, lays the echograms of all queries out in
+ * one device arena, splits the rays into work items of 64 consecutive
+ * rays of one source and captures clear + persistent launch in a
+ * hipGraph.  Each query is validated as omg_trace_world validates it.
+ * The scenes must stay alive until omg_batch_destroy. */
+int omg_batch_create(omg_ctx *ctx, const omg_query *queries, uint32_t n_queries, uint32_t flags,
+                     omg_batch **out);
+/* Replays the graph on `stream` (NULL = the context's stream) without waiting. */
+int omg_batch_launch(omg_batch *batch, void *stream);
+int omg_batch_sync(omg_batch *batch);
+/* Copies query q's echograms (and records, with OMG_BATCH_RECORDS) to host
+ * buffers laid out as omg_trace_world's; waits for the last launch. */
+int omg_batch_result(omg_batch *batch, uint32_t q, float *bins, float *dirs,
+                     omg_ray_record *records);
+/* Device addresses of query q's echograms (valid until omg_batch_destroy). */
+int omg_batch_device_output(omg_batch *batch, uint32_t q, float **d_bins, float **d_dirs);
+int omg_batch_counters(omg_batch *batch, omg_counters *counters);
+/* Total rays and work items of the batch. */
+int omg_batch_size(omg_batch *batch, uint64_t *rays, uint32_t *items);
+int omg_batch_get_shape(omg_batch *batch, omg_batch_shape *shape);
+int omg_batch_set_shape(omg_batch *batch, const omg_batch_shape *shape);
+/* Autotuner: times every launch shape on a strided sample of the batch's
+ * work items (every `stride`-th), keeps the fastest, re-captures the graph
+ * and caches the choice on the context for batches of the same class
+ * (widest band count, total rays and mean BVH size, each to a power of 2).
+ * out may be NULL.  Outputs of the sampled launches are overwritten by
+ * the next omg_batch_launch. */
+int omg_batch_tune(omg_batch *batch, uint32_t stride, omg_batch_shape *out);
+void omg_batch_destroy(omg_batch *batch);
+
 /* ------------------------------------------------------------ echogram post-processing */
 
 /* estimate_reverb (tracer.rs:205-257): rt60[nbands], level[nbands]. */

@@ -41,6 +41,7 @@ extern "C" int omg_init(int device, uint32_t layout_version, omg_ctx** out)
     omg_ctx* c = new omg_ctx();
     c->device = device;
     c->name = prop.gcnArchName;
+    c->n_cu = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 1u;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -258,6 +259,11 @@ extern "C" void omg_scene_destroy(omg_scene* s)
     s->tri_surf.release();
     omg_ctx* c = s->ctx;
     delete s;
+    ctx_release(c);
+}
+
+void omg::ctx_release(omg_ctx* c)
+{
     c->live_scenes--;
     if (c->closing && c->live_scenes == 0) ctx_teardown(c);
 }
@@ -265,11 +271,11 @@ extern "C" void omg_scene_destroy(omg_scene* s)
 // ------------------------------------------------------------------ tracer
 
 namespace {
-
 uint32_t nbins_of(float max_time, float bin_dt) { return sat_u32(max_time / bin_dt); }
+}  // namespace
 
-int validate_params(const omg_trace_params* p, uint32_t nbands, uint32_t n_src, uint32_t n_lis,
-                    uint32_t n_ovl, uint32_t* nbins)
+int omg::validate_params(const omg_trace_params* p, uint32_t nbands, uint32_t n_src,
+                         uint32_t n_lis, uint32_t n_ovl, uint32_t* nbins)
 {
     if (!p) return fail(OMG_ERR_INVALID, "params is null");
     if (p->nbands != nbands) return fail(OMG_ERR_INVALID, "params.nbands differs from the scene");
@@ -286,13 +292,14 @@ int validate_params(const omg_trace_params* p, uint32_t nbands, uint32_t n_src, 
     return OMG_OK;
 }
 
-// Fills the launch-invariant fields and uploads the job's material table
-// (scene materials, then overlay materials) and overlays.
-int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatParams>& base_mats,
-                const omg_overlay* ovl, uint32_t n_ovl, uint32_t nbands, TraceArgs& A)
+// The job's material table (scene materials, then one per overlay) and
+// its overlay boxes.
+void omg::job_tables(const std::vector<MatParams>& base_mats, const omg_overlay* ovl,
+                     uint32_t n_ovl, uint32_t nbands, std::vector<MatParams>& mats,
+                     std::vector<Overlay>& ov)
 {
-    std::vector<MatParams> mats(base_mats);
-    std::vector<Overlay> ov(n_ovl);
+    mats = base_mats;
+    ov.assign(n_ovl, Overlay{});
     for (uint32_t i = 0; i < n_ovl; i++) {
         MatParams mp;
         mat_params(ovl[i].mat.absorption, ovl[i].mat.scattering, ovl[i].mat.transmission, nbands, mp);
@@ -304,13 +311,12 @@ int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatPara
         ov[i].mat = (uint32_t)(base_mats.size() + i);
         ov[i].pad = 0;
     }
-    int rc = upload(c->job_mats, mats.data(), mats.size() * sizeof(MatParams), c->stream);
-    if (!rc) rc = upload(c->job_ovl, ov.data(), ov.size() * sizeof(Overlay), c->stream);
-    // the staging vectors die with this frame: wait for the copies to land
-    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "job upload");
-    if (rc) return rc;
-    A.mats = c->job_mats.as<MatParams>();
-    A.overlays = c->job_ovl.as<Overlay>();
+}
+
+// Launch-invariant scalars of a job (tracer.rs:92-103 with the constants
+// of tracer.rs:12-16 as parameters).
+void omg::job_scalars(const omg_trace_params* p, uint32_t nbands, uint32_t n_ovl, JobView& A)
+{
     A.n_ovl = n_ovl;
     A.n_rays = p->n_rays;
     A.nbands = nbands;
@@ -324,6 +330,25 @@ int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatPara
     float per_ray = 1.0f / (float)p->n_rays;
     A.floor_e = p->floor_rel * per_ray;
     for (int b = 0; b < kMaxBands; b++) A.energy0[b] = (uint32_t)b < nbands ? per_ray * p->energy[b] : 0.0f;
+}
+
+namespace {
+
+// Uploads the job's material table and overlays and fills the job scalars.
+int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatParams>& base_mats,
+                const omg_overlay* ovl, uint32_t n_ovl, uint32_t nbands, TraceArgs& A)
+{
+    std::vector<MatParams> mats;
+    std::vector<Overlay> ov;
+    job_tables(base_mats, ovl, n_ovl, nbands, mats, ov);
+    int rc = upload(c->job_mats, mats.data(), mats.size() * sizeof(MatParams), c->stream);
+    if (!rc) rc = upload(c->job_ovl, ov.data(), ov.size() * sizeof(Overlay), c->stream);
+    // the staging vectors die with this frame: wait for the copies to land
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "job upload");
+    if (rc) return rc;
+    A.mats = c->job_mats.as<MatParams>();
+    A.overlays = c->job_ovl.as<Overlay>();
+    job_scalars(p, nbands, n_ovl, A);
     return OMG_OK;
 }
 

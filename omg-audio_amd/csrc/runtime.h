@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -60,13 +61,17 @@ struct omg_ctx {
     // dome fan of the last size used, and the sticky overflow word of async launches
     omg::DevBuf fan_dirs, fan_bins, dome_status, dome_panels;
     uint32_t fan_n = 0;
-    // scenes alive on this context; omg_shutdown defers the teardown until the last is destroyed
+    // scenes and batches alive on this context; omg_shutdown defers the
+    // teardown until the last is destroyed
     uint32_t live_scenes = 0;
+    // autotuned batch launch shapes by batch class
+    std::map<uint64_t, omg::BatchShape> tuned;
     // tracer launch configuration (omg_set_launch_config)
     uint32_t node_cache = 512;
     uint32_t stack_lds = 16;
     uint32_t lanes = 0;                 // 0: pick per launch (pick_lanes)
-    uint64_t quad_below = 0;            // launches with fewer rays trace a quad per ray
+    uint64_t quad_below = 1ull << 20;   // launches with fewer rays trace a quad per ray
+    uint32_t n_cu = 0;                  // compute units (persistent launches size their grid to it)
     bool closing = false;
 };
 
@@ -94,4 +99,11 @@ hipError_t launch_fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s);
 uint32_t pick_lanes(const omg_ctx* c, uint64_t n_rays);
 void fill_geom(const omg_scene* s, GeomArgs& g);
+int validate_params(const omg_trace_params* p, uint32_t nbands, uint32_t n_src, uint32_t n_lis,
+                    uint32_t n_ovl, uint32_t* nbins);
+void job_tables(const std::vector<MatParams>& base_mats, const omg_overlay* ovl, uint32_t n_ovl,
+                uint32_t nbands, std::vector<MatParams>& mats, std::vector<Overlay>& ov);
+void job_scalars(const omg_trace_params* p, uint32_t nbands, uint32_t n_ovl, JobView& A);
+// drops one scene/batch reference; tears the context down after omg_shutdown
+void ctx_release(omg_ctx* c);
 }  // namespace omg

@@ -5,7 +5,7 @@
 
 namespace omg {
 
-// Geometry of one launch: a flattened mesh BVH, or the analytic Shoebox.
+// Geometry of one job: a flattened mesh BVH, or the analytic Shoebox.
 struct GeomArgs {
     const Node2* nodes;
     const Prim* prims;
@@ -18,7 +18,9 @@ struct GeomArgs {
     float root_max[3];
 };
 
-struct TraceArgs {
+// Everything one tracer job (one trace_world call, or one query of a batch)
+// needs per ray.
+struct JobView {
     GeomArgs g;
     const MatParams* mats;   // scene materials, then overlay materials (box: 6 walls first)
     const Overlay* overlays;
@@ -29,11 +31,14 @@ struct TraceArgs {
     uint32_t n_rays, nbands, max_bounces, dir_band;
     uint32_t key0, key1;     // Philox key = job seed
     float bin_dt, max_time, floor_e, radius;
-    float energy0[kMaxBands];  // per_ray * source energy (tracer.rs:95)
+    float energy0[kMaxBands];  // per_ray * source energy (tracer.rs:95); 0 beyond nbands
     uint32_t nbins;
     float* bins;             // [n_src][n_lis][nbins][nbands]
     float* dirs;             // [n_src][n_lis][nbins][3]
     uint32_t* records;       // [n_src * n_rays][4] or null
+};
+
+struct TraceArgs : JobView {
     unsigned long long* counters;  // [3] or null
     uint32_t ray_base;       // first global ray of this launch (chunked launches)
     uint32_t ray_count;      // rays in this launch
@@ -43,5 +48,38 @@ struct TraceArgs {
     uint32_t* hit_ids;       // [rays][max_bounces] hit id per bounce (0xFFFFFFFF none), or null
     float* hit_t;            // [rays][max_bounces] hit distance per bounce, or null
 };
+
+// One unit of batched work: up to kBatchItemRays consecutive rays of one
+// source of one query.
+constexpr uint32_t kBatchItemRays = 64;
+
+struct BatchItem {
+    uint32_t query;
+    uint32_t src;
+    uint32_t ray_start;
+    uint32_t count;
+};
+
+struct BatchArgs {
+    const JobView* jobs;     // [n_queries]
+    const BatchItem* items;  // [n_items]
+    uint32_t n_items;
+    uint32_t* next_item;     // work-stealing cursor, zero before each launch
+    unsigned long long* counters;  // [3] or null
+};
+
+// Launch shape of the batched kernel (chosen by the autotuner).
+struct BatchShape {
+    uint32_t lanes;           // lanes per ray: 1 or 4
+    uint32_t stack_lds;       // 16 or 32 stack entries per ray in LDS
+    uint32_t waves_per_simd;  // resident waves per SIMD the grid is sized for
+    uint32_t regen;           // 1: dead lanes take new rays between bounces (lanes = 1 only)
+};
+
+hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s);
+// persistent work-stealing launch over n_cu compute units; nbm = the largest
+// band count of the batch
+hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape& shape,
+                              uint32_t n_cu, bool count, hipStream_t s);
 
 }  // namespace omg

@@ -51,7 +51,7 @@ __device__ __forceinline__ bool box_raycast_hit(const GeomArgs& G, f3 p, f3 d, H
 }
 
 // WithPanels::raycast_hit overlay loop (scene.rs:99-135)
-__device__ __forceinline__ void overlay_hits(const TraceArgs& A, f3 p, f3 d, bool& found, Hit& h)
+__device__ __forceinline__ void overlay_hits(const JobView& A, f3 p, f3 d, bool& found, Hit& h)
 {
     for (uint32_t i = 0; i < A.n_ovl; i++) {
         const Overlay ov = A.overlays[i];
@@ -99,7 +99,7 @@ __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const QuadTr
 }
 
 template <bool MESH, bool COUNT, int SK, class Trav>
-__device__ __forceinline__ bool raycast_hit(const TraceArgs& A, const Trav& L, f3 p, f3 d,
+__device__ __forceinline__ bool raycast_hit(const JobView& A, const Trav& L, f3 p, f3 d,
                                             Hit& h, TravCount& c)
 {
     bool found;
@@ -153,9 +153,184 @@ __device__ __forceinline__ void init_trav(QuadTrav& L, lds_float4* lds, uint32_t
     L.q = threadIdx.x & 3u;
 }
 
-// tracer::trace (tracer.rs:80-202).  LANES = 1: one lane per (source, ray);
-// LANES = 4: a DPP quad per ray (see mesh_raycast_quad), the quad's lanes
-// also splitting the listeners between them.
+// State of one ray between bounces (tracer.rs:92-102).
+template <int NBM>
+struct RayState {
+    f3 pos, dir;
+    float energy[NBM];
+    float dist_total;
+    uint32_t bounce;  // bounce-loop iterations so far
+    uint32_t term, hash, rflags;
+    uint32_t src, ray;
+};
+
+// Ray start (tracer.rs:92-102): source position, uniform direction from
+// the Philox block (ray, 0, source), energy per band.
+template <int NBM>
+__device__ __forceinline__ void ray_begin(const JobView& A, uint32_t src, uint32_t ray,
+                                          RayState<NBM>& S)
+{
+    S.src = src;
+    S.ray = ray;
+    S.pos = mk(A.sources[3 * src], A.sources[3 * src + 1], A.sources[3 * src + 2]);
+    uint32_t w[4];
+    philox4x32_10(ray, 0u, src, 0u, A.key0, A.key1, w);
+    S.dir = unit_sphere(u24(w[0]), u24(w[1]));
+#pragma unroll
+    for (int b = 0; b < NBM; b++) S.energy[b] = A.energy0[b];
+    S.dist_total = 0.0f;
+    S.bounce = 0;
+    S.term = 4u;
+    S.hash = 2166136261u;
+    S.rflags = 0;
+}
+
+// One iteration of the bounce loop (tracer.rs:103-200): raycast, score every
+// listener the segment passes (tracer.rs:110-128) into the job's
+// echograms, then escape / time cut-off / transmission / absorption /
+// scattering.  Returns false when the ray has ended (S.term says why; 4 =
+// the bounce limit).  NBM is the compile-time band capacity, nb the job's
+// band count (<= NBM; energies beyond it are zero and never scored).
+// LANES = 4 runs the raycast on a DPP quad (mesh_raycast_quad); the quad's
+// lanes then split the listeners between them.
+template <int NBM, bool MESH, bool COUNT, int SK, int LANES, class Trav>
+__device__ __forceinline__ bool ray_step(const JobView& A, const Trav& L, uint32_t q, uint32_t nb,
+                                         RayState<NBM>& S, uint32_t* hit_ids, float* hit_t,
+                                         TravCount& tc)
+{
+    const uint32_t bounce = S.bounce;
+    S.bounce = bounce + 1u;
+    Hit h;
+    bool has = raycast_hit<MESH, COUNT, SK>(A, L, S.pos, S.dir, h, tc);
+    float t_hit = has ? h.t : FLT_MAX;
+    const size_t eg = (size_t)A.nbins * nb, dg = (size_t)A.nbins * 3;
+    for (uint32_t li = q; li < A.n_lis; li += LANES) {
+        f3 Lp = mk(A.listeners[3 * li], A.listeners[3 * li + 1], A.listeners[3 * li + 2]);
+        f3 to_l = sub(Lp, S.pos);
+        float s = dot(to_l, S.dir);
+        if (s > 0.0f && s < t_hit) {
+            f3 closest = add(S.pos, mul(S.dir, s));
+            if (length(sub(closest, Lp)) < A.radius) {
+                float arrival = (S.dist_total + s) / kSpeedOfSound;
+                uint32_t bin = sat_u32(arrival / A.bin_dt);
+                if (bin < A.nbins) {
+                    size_t pair = (size_t)S.src * A.n_lis + li;
+                    float* ob = A.bins + pair * eg + (size_t)bin * nb;
+                    float* od = A.dirs + pair * dg + (size_t)bin * 3;
+#pragma unroll
+                    for (int b = 0; b < NBM; b++)
+                        if ((uint32_t)b < nb) unsafeAtomicAdd(ob + b, S.energy[b]);
+                    float ed = S.energy[0];
+#pragma unroll
+                    for (int b = 1; b < NBM; b++)
+                        if ((uint32_t)b == A.dir_band) ed = S.energy[b];
+                    unsafeAtomicAdd(od + 0, -(S.dir.x * ed));
+                    unsafeAtomicAdd(od + 1, -(S.dir.y * ed));
+                    unsafeAtomicAdd(od + 2, -(S.dir.z * ed));
+                }
+            }
+        }
+    }
+    if (hit_ids && q == 0) {
+        size_t slot = ((size_t)S.src * A.n_rays + S.ray) * A.max_bounces + bounce;
+        hit_ids[slot] = has ? h.id : 0xFFFFFFFFu;
+        hit_t[slot] = has ? h.t : 0.0f;
+    }
+    if (!has) {
+        S.term = 0u;
+        return false;
+    }
+    S.hash = fnv1a_u32(S.hash, h.id);
+    S.rflags |= h.flags;
+    S.pos = add(S.pos, mul(S.dir, t_hit));
+    S.dist_total += t_hit;
+    if (S.dist_total / kSpeedOfSound > A.max_time) {
+        S.term = 1u;
+        return false;
+    }
+    const MatParams* M = A.mats + h.mat;
+    uint32_t w[4];
+    philox4x32_10(S.ray, bounce + 1u, S.src, 0u, A.key0, A.key1, w);
+    float p = M->p;
+    bool transmitted = p > 0.0f && u24(w[0]) < p;
+    bool alive = false;
+    if (transmitted) {
+#pragma unroll
+        for (int b = 0; b < NBM; b++) {
+            S.energy[b] *= M->tw[b];
+            if ((uint32_t)b < nb && S.energy[b] > A.floor_e) alive = true;
+        }
+        if (!alive) {
+            S.term = 2u;
+            return false;
+        }
+        S.pos = add(S.pos, mul(S.dir, 2.0f * 1e-3f));
+        return S.bounce < A.max_bounces;
+    }
+#pragma unroll
+    for (int b = 0; b < NBM; b++) {
+        S.energy[b] *= M->rw[b];
+        if ((uint32_t)b < nb && S.energy[b] > A.floor_e) alive = true;
+    }
+    if (!alive) {
+        S.term = 3u;
+        return false;
+    }
+    f3 n = h.n;
+    if (u24(w[1]) < M->scattering) {
+        S.dir = normalize(add(n, unit_sphere(u24(w[2]), u24(w[3]))));
+        if (dot(S.dir, n) < 1e-3f) S.dir = n;
+    } else {
+        S.dir = sub(S.dir, mul(n, 2.0f * dot(S.dir, n)));
+    }
+    S.pos = add(S.pos, mul(n, 1e-3f));
+    return S.bounce < A.max_bounces;
+}
+
+// Per-ray parity record: bounce-loop count, termination, FNV-1a of the hit
+// ids, near-edge / tie flags.
+template <int NBM>
+__device__ __forceinline__ void ray_end(const JobView& A, const RayState<NBM>& S, uint32_t q)
+{
+    if (A.records && q == 0) {
+        uint4 r = make_uint4(S.bounce, S.term, S.hash, S.rflags);
+        reinterpret_cast<uint4*>(A.records)[(size_t)S.src * A.n_rays + S.ray] = r;
+    }
+}
+
+// tracer::trace (tracer.rs:80-202) for one ray of one source, start to end
+// (max_bounces >= 1).  Returns the bounce-loop count.
+template <int NBM, bool MESH, bool COUNT, int SK, int LANES, class Trav>
+__device__ __forceinline__ uint32_t trace_ray(const JobView& A, const Trav& L, uint32_t q,
+                                              uint32_t src, uint32_t ray, uint32_t nb,
+                                              uint32_t* hit_ids, float* hit_t, TravCount& tc)
+{
+    RayState<NBM> S;
+    ray_begin(A, src, ray, S);
+    while (ray_step<NBM, MESH, COUNT, SK, LANES>(A, L, q, nb, S, hit_ids, hit_t, tc)) {
+    }
+    ray_end(A, S, q);
+    return S.bounce;
+}
+
+// Wave-level reduction of the traversal counters, one atomic per wave and counter.
+__device__ __forceinline__ void flush_counters(unsigned long long* counters, uint64_t aabb,
+                                               uint64_t tri, uint64_t iters)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        aabb += __shfl_xor(aabb, off);
+        tri += __shfl_xor(tri, off);
+        iters += __shfl_xor(iters, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(counters + 0, (unsigned long long)aabb);
+        atomicAdd(counters + 1, (unsigned long long)tri);
+        atomicAdd(counters + 2, (unsigned long long)iters);
+    }
+}
+
+// Single-job launch: thread -> (source, ray) in source-major order.
 //
 // Dynamic LDS: the first nc BVH nodes (breadth-first, so the top levels),
 // then the SK-deep stack tops of the block's rays.  Blocks are dispatched
@@ -177,121 +352,132 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     uint32_t blk = (nb & 7u) == 0 ? xcd * per + (blockIdx.x >> 3) : blockIdx.x;
     uint32_t lid = (blk * blockDim.x + threadIdx.x) / LANES;
     const uint32_t q = LANES == 1 ? 0u : (threadIdx.x & (LANES - 1));
-    bool active = lid < A.ray_count;
     uint64_t aabb_acc = 0, tri_acc = 0, iters_acc = 0;
-    if (active) {
+    if (lid < A.ray_count) {
         uint32_t gid = A.ray_base + lid;
         uint32_t src = gid / A.n_rays;
         uint32_t ray = gid - src * A.n_rays;
-        f3 pos = mk(A.sources[3 * src], A.sources[3 * src + 1], A.sources[3 * src + 2]);
-        uint32_t w[4];
-        philox4x32_10(ray, 0u, src, 0u, A.key0, A.key1, w);
-        f3 dir = unit_sphere(u24(w[0]), u24(w[1]));
-        float energy[NBM];
-#pragma unroll
-        for (int b = 0; b < NBM; b++) energy[b] = A.energy0[b];
-        float dist_total = 0.0f;
-        uint32_t iters = 0, term = 4u, hash = 2166136261u, rflags = 0;
-        const size_t eg = (size_t)A.nbins * NBM, dg = (size_t)A.nbins * 3;
-        float* ob_src = A.bins + (size_t)src * A.n_lis * eg;
-        float* od_src = A.dirs + (size_t)src * A.n_lis * dg;
         TravCount tc;
-        for (uint32_t bounce = 0; bounce < A.max_bounces; bounce++) {
-            iters++;
-            Hit h;
-            bool has = raycast_hit<MESH, COUNT, SK>(A, L, pos, dir, h, tc);
-            float t_hit = has ? h.t : FLT_MAX;
-            // receiver sphere crossing along this segment (tracer.rs:110-128)
-            for (uint32_t li = q; li < A.n_lis; li += LANES) {
-                f3 Lp = mk(A.listeners[3 * li], A.listeners[3 * li + 1], A.listeners[3 * li + 2]);
-                f3 to_l = sub(Lp, pos);
-                float s = dot(to_l, dir);
-                if (s > 0.0f && s < t_hit) {
-                    f3 closest = add(pos, mul(dir, s));
-                    if (length(sub(closest, Lp)) < A.radius) {
-                        float arrival = (dist_total + s) / kSpeedOfSound;
-                        uint32_t bin = sat_u32(arrival / A.bin_dt);
-                        if (bin < A.nbins) {
-                            float* ob = ob_src + li * eg + (size_t)bin * NBM;
-                            float* od = od_src + li * dg + (size_t)bin * 3;
-#pragma unroll
-                            for (int b = 0; b < NBM; b++) unsafeAtomicAdd(ob + b, energy[b]);
-                            float ed = energy[0];
-#pragma unroll
-                            for (int b = 1; b < NBM; b++)
-                                if ((uint32_t)b == A.dir_band) ed = energy[b];
-                            unsafeAtomicAdd(od + 0, -(dir.x * ed));
-                            unsafeAtomicAdd(od + 1, -(dir.y * ed));
-                            unsafeAtomicAdd(od + 2, -(dir.z * ed));
-                        }
-                    }
-                }
-            }
-            if (A.hit_ids && q == 0) {
-                size_t slot = (size_t)gid * A.max_bounces + bounce;
-                A.hit_ids[slot] = has ? h.id : 0xFFFFFFFFu;
-                A.hit_t[slot] = has ? h.t : 0.0f;
-            }
-            if (!has) { term = 0u; break; }
-            hash = fnv1a_u32(hash, h.id);
-            rflags |= h.flags;
-            pos = add(pos, mul(dir, t_hit));
-            dist_total += t_hit;
-            if (dist_total / kSpeedOfSound > A.max_time) { term = 1u; break; }
-            const MatParams* M = A.mats + h.mat;
-            philox4x32_10(ray, bounce + 1u, src, 0u, A.key0, A.key1, w);
-            float p = M->p;
-            bool transmitted = p > 0.0f && u24(w[0]) < p;
-            bool alive = false;
-            if (transmitted) {
-#pragma unroll
-                for (int b = 0; b < NBM; b++) {
-                    energy[b] *= M->tw[b];
-                    if (energy[b] > A.floor_e) alive = true;
-                }
-                if (!alive) { term = 2u; break; }
-                pos = add(pos, mul(dir, 2.0f * 1e-3f));
-                continue;
-            }
-#pragma unroll
-            for (int b = 0; b < NBM; b++) {
-                energy[b] *= M->rw[b];
-                if (energy[b] > A.floor_e) alive = true;
-            }
-            if (!alive) { term = 3u; break; }
-            f3 n = h.n;
-            if (u24(w[1]) < M->scattering) {
-                dir = normalize(add(n, unit_sphere(u24(w[2]), u24(w[3]))));
-                if (dot(dir, n) < 1e-3f) dir = n;
-            } else {
-                dir = sub(dir, mul(n, 2.0f * dot(dir, n)));
-            }
-            pos = add(pos, mul(n, 1e-3f));
-        }
-        if (A.records && q == 0) {
-            uint4 r = make_uint4(iters, term, hash, rflags);
-            reinterpret_cast<uint4*>(A.records)[gid] = r;
-        }
+        uint32_t iters = trace_ray<NBM, MESH, COUNT, SK, LANES>(A, L, q, src, ray, (uint32_t)NBM,
+                                                                A.hit_ids, A.hit_t, tc);
         if (COUNT && q == 0) {
             aabb_acc = tc.aabb;
             tri_acc = tc.tri;
             iters_acc = iters;
         }
     }
-    if (COUNT && A.counters) {
-        // wave-level reduction, one atomic per wave and counter
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            aabb_acc += __shfl_xor(aabb_acc, off);
-            tri_acc += __shfl_xor(tri_acc, off);
-            iters_acc += __shfl_xor(iters_acc, off);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(A.counters + 0, (unsigned long long)aabb_acc);
-            atomicAdd(A.counters + 1, (unsigned long long)tri_acc);
-            atomicAdd(A.counters + 2, (unsigned long long)iters_acc);
+    if (COUNT && A.counters) flush_counters(A.counters, aabb_acc, tri_acc, iters_acc);
+}
+
+// Batched persistent launch: the grid is sized to the resident wave slots and
+// every wave steals work items (kBatchItemRays consecutive rays of one source
+// of one query) from a global cursor until the list is exhausted.  Queries
+// of different scenes, sizes and band counts share the launch; each ray runs
+// the same trace_ray as the single-job kernel, so its path and per-ray
+// record are bit-identical to the query's own omg_trace_world.  No BVH
+// nodes are staged in LDS (a wave moves between scenes); the LDS holds the
+// SK-deep stack tops only.
+template <int NBM, bool COUNT, int SK, int LANES>
+__global__ __launch_bounds__(256) void k_trace_batch(BatchArgs B)
+{
+    extern __shared__ float4 lds[];
+    typename TravOf<LANES>::type L;
+    init_trav(L, (lds_float4*)lds, 0u, SK);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t q = LANES == 1 ? 0u : (threadIdx.x & (LANES - 1));
+    constexpr uint32_t kPerPass = 64u / LANES;
+    uint64_t aabb_acc = 0, tri_acc = 0, iters_acc = 0;
+    for (;;) {
+        // every lane of the wave is active here: lane 0 is the first active lane
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(B.next_item, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= B.n_items) break;
+        const BatchItem item = B.items[it];
+        const JobView& J = B.jobs[item.query];
+        for (uint32_t base = 0; base < item.count; base += kPerPass) {
+            uint32_t lid = base + lane / LANES;
+            if (lid < item.count) {
+                TravCount tc;
+                uint32_t iters = trace_ray<NBM, true, COUNT, SK, LANES>(
+                    J, L, q, item.src, item.ray_start + lid, J.nbands, nullptr, nullptr, tc);
+                if (COUNT && q == 0) {
+                    aabb_acc += tc.aabb;
+                    tri_acc += tc.tri;
+                    iters_acc += iters;
+                }
+            }
         }
     }
+    if (COUNT && B.counters) flush_counters(B.counters, aabb_acc, tri_acc, iters_acc);
+}
+
+// Batched persistent launch with per-lane ray regeneration: a lane whose
+// ray has ended takes the next ray of its wave's current work item between
+// bounces, so a wave's lanes stay busy while the rays' bounce counts differ.
+// Dead lanes are ranked by a ballot (mbcnt over the dead-lane mask) and
+// given consecutive rays; the wave takes a new item from the cursor when
+// its item is used up.  Lanes of one wave may then hold rays of different
+// queries: the job view is read per lane.  One lane per ray.
+template <int NBM, bool COUNT, int SK>
+__global__ __launch_bounds__(256) void k_trace_batch_regen(BatchArgs B)
+{
+    extern __shared__ float4 lds[];
+    LdsTrav L;
+    init_trav(L, (lds_float4*)lds, 0u, SK);
+    const uint32_t lane = threadIdx.x & 63u;
+    RayState<NBM> S;
+    uint32_t live = 0, qi = 0;
+    // the wave's current item: rays [it_next, it_end) of (it_q, it_src); uniform
+    uint32_t it_q = 0, it_src = 0, it_next = 0, it_end = 0, more = 1;
+    uint64_t aabb_acc = 0, tri_acc = 0, iters_acc = 0;
+    for (;;) {
+        // every lane of the wave is active here: lane 0 is the first active lane
+        uint64_t need = __ballot(live == 0u);
+        while (need != 0 && (it_next < it_end || more)) {
+            if (it_next == it_end) {
+                uint32_t it = 0;
+                if (lane == 0) it = atomicAdd(B.next_item, 1u);
+                it = __builtin_amdgcn_readfirstlane(it);
+                if (it >= B.n_items) {
+                    more = 0;
+                    break;
+                }
+                const BatchItem I = B.items[it];
+                it_q = I.query;
+                it_src = I.src;
+                it_next = I.ray_start;
+                it_end = I.ray_start + I.count;
+            }
+            const uint32_t avail = it_end - it_next;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+            if (live == 0u && rank < avail) {
+                ray_begin(B.jobs[it_q], it_src, it_next + rank, S);
+                qi = it_q;
+                live = 1u;
+            }
+            const uint32_t k = (uint32_t)__popcll(need);
+            it_next += k < avail ? k : avail;
+            need = __ballot(live == 0u);
+        }
+        if (__ballot(live != 0u) == 0) break;
+        if (live) {
+            const JobView& J = B.jobs[qi];
+            TravCount tc;
+            bool cont = ray_step<NBM, true, COUNT, SK, 1>(J, L, 0u, J.nbands, S, nullptr, nullptr, tc);
+            if (COUNT) {
+                aabb_acc += tc.aabb;
+                tri_acc += tc.tri;
+            }
+            if (!cont) {
+                ray_end(J, S, 0u);
+                if (COUNT) iters_acc += S.bounce;
+                live = 0u;
+            }
+        }
+    }
+    if (COUNT && B.counters) flush_counters(B.counters, aabb_acc, tri_acc, iters_acc);
 }
 
 __global__ void k_fill_zero(float* p, size_t n)
@@ -380,6 +566,59 @@ hipError_t launch_fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s)
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_zero_u64, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, p, n);
     return hipGetLastError();
+}
+
+}  // namespace omg
+
+namespace omg {
+
+template <int NBM, bool COUNT, int SK, int LANES>
+static hipError_t launch_batch_l(const BatchArgs& B, uint32_t blocks, hipStream_t s)
+{
+    size_t lds = (size_t)SK * (kTraceBlock / LANES) * 8;
+    hipLaunchKernelGGL((k_trace_batch<NBM, COUNT, SK, LANES>), dim3(blocks), dim3(kTraceBlock), lds,
+                       s, B);
+    return hipGetLastError();
+}
+
+template <int NBM, bool COUNT, int SK>
+static hipError_t launch_batch_regen(const BatchArgs& B, uint32_t blocks, hipStream_t s)
+{
+    size_t lds = (size_t)SK * kTraceBlock * 8;
+    hipLaunchKernelGGL((k_trace_batch_regen<NBM, COUNT, SK>), dim3(blocks), dim3(kTraceBlock), lds,
+                       s, B);
+    return hipGetLastError();
+}
+
+template <int NBM, bool COUNT>
+static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint32_t blocks,
+                                  hipStream_t s)
+{
+    if (sh.regen)
+        return sh.stack_lds >= 32 ? launch_batch_regen<NBM, COUNT, 32>(B, blocks, s)
+                                  : launch_batch_regen<NBM, COUNT, 16>(B, blocks, s);
+    if (sh.stack_lds >= 32)
+        return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 32, 4>(B, blocks, s)
+                             : launch_batch_l<NBM, COUNT, 32, 1>(B, blocks, s);
+    return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 16, 4>(B, blocks, s)
+                         : launch_batch_l<NBM, COUNT, 16, 1>(B, blocks, s);
+}
+
+// Band capacities compiled for batches: 3, 6 and 8 (a batch runs at the
+// smallest one holding its widest query).
+hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape& sh,
+                              uint32_t n_cu, bool count, hipStream_t s)
+{
+    if (B.n_items == 0) return hipSuccess;
+    if (sh.lanes != 1 && sh.lanes != 4) return hipErrorInvalidValue;
+    if (sh.regen && sh.lanes != 1) return hipErrorInvalidValue;
+    if (sh.waves_per_simd == 0 || sh.waves_per_simd > 8) return hipErrorInvalidValue;
+    // 4 SIMDs per CU and 4 waves per 256-lane block: one block per CU per wave slot
+    uint32_t blocks = n_cu * sh.waves_per_simd;
+    if (nbm <= 3) return count ? launch_batch_nb<3, true>(B, sh, blocks, s) : launch_batch_nb<3, false>(B, sh, blocks, s);
+    if (nbm <= 6) return count ? launch_batch_nb<6, true>(B, sh, blocks, s) : launch_batch_nb<6, false>(B, sh, blocks, s);
+    if (nbm <= 8) return count ? launch_batch_nb<8, true>(B, sh, blocks, s) : launch_batch_nb<8, false>(B, sh, blocks, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace omg

@@ -383,3 +383,6 @@ from .early import GpuWorldDiscovery, GpuWorldSolve, surface_table  # noqa: E402
 
 __all__ += ["DomeProbe", "dome_params", "dome_sample", "dome_sample_async", "dome_status", "GpuWorldDiscovery", "GpuWorldSolve",
             "surface_table"]
+from .batch import Batch, QuerySpec  # noqa: E402
+
+__all__ += ["Batch", "QuerySpec"]

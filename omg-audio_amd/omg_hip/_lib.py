@@ -90,10 +90,19 @@ class MeshRecord(C.Structure):
                 ("gains", F8)]
 
 
-class BatchQuery(C.Structure):
+class Query(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("params", TraceParams), ("sources", C.c_void_p),
                 ("n_src", C.c_uint32), ("listeners", C.c_void_p), ("n_lis", C.c_uint32),
-                ("bins", C.c_void_p), ("dirs", C.c_void_p)]
+                ("overlays", C.c_void_p), ("n_ovl", C.c_uint32)]
+
+
+class BatchShape(C.Structure):
+    _fields_ = [("lanes", C.c_uint32), ("stack_lds", C.c_uint32), ("waves_per_simd", C.c_uint32),
+                ("regen", C.c_uint32), ("ms", C.c_float)]
+
+
+BATCH_RECORDS = 1
+BATCH_COUNTERS = 2
 
 
 # (name, restype, argtypes) of every exported entry point
@@ -120,6 +129,17 @@ SIGNATURES = [
     ("omg_trace_box", _i, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("omg_trace_world_async", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp,
                                    _vp]),
+    ("omg_batch_create", _i, [_vp, _vp, _u32, _u32, C.POINTER(_vp)]),
+    ("omg_batch_launch", _i, [_vp, _vp]),
+    ("omg_batch_sync", _i, [_vp]),
+    ("omg_batch_result", _i, [_vp, _u32, _vp, _vp, _vp]),
+    ("omg_batch_device_output", _i, [_vp, _u32, C.POINTER(_vp), C.POINTER(_vp)]),
+    ("omg_batch_counters", _i, [_vp, _vp]),
+    ("omg_batch_size", _i, [_vp, C.POINTER(C.c_uint64), C.POINTER(_u32)]),
+    ("omg_batch_get_shape", _i, [_vp, _vp]),
+    ("omg_batch_set_shape", _i, [_vp, _vp]),
+    ("omg_batch_tune", _i, [_vp, _u32, _vp]),
+    ("omg_batch_destroy", None, [_vp]),
     ("omg_estimate_reverb", _i, [_vp, _u32, _u32, _f32, _vp, _vp]),
     ("omg_late_direction", _i, [_vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp]),
     ("omg_dome_sample", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),

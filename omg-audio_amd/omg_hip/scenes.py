@@ -277,3 +277,46 @@ def outdoor_points(seed, n, bounds, boxes, z=(1.2, 1.8), margin=0.6):
             continue
         out.append(p)
     return np.asarray(out, np.float32)
+
+
+def c3_workload(seed=3, n_queries=256, n_scenes=32, tris=(1_000, 200_000), sources=(1, 32),
+                rays=(1024, 65536), listeners=(1, 4), bands=(3, 6), seconds=(1, 4)):
+    """Batched mixed-size workload: n_queries queries over a pool of
+    n_scenes procedural blocks.
+
+    Scene i: a city block with triangle target log-U[tris], 3 or 6 bands.
+    Query: a pool scene, sources round(log-U[sources]) and rays per source
+    round(log-U[rays]), listeners U{listeners}, half the points inside
+    buildings and half in the streets, echogram 1 ms x U{seconds} s, its own
+    Philox seed.  Returns (scene_specs, queries): scene_specs[i] =
+    (builder, nbands), queries[j] = dict(scene, n_src, n_rays, nbands,
+    max_time, seed, sources, listeners).
+    """
+    rng = SplitMix64(seed)
+    specs, geo = [], []
+    for i in range(n_scenes):
+        t = int(round(rng.log_uniform(*tris)))
+        nb = bands[rng.randint(0, len(bands) - 1)]
+        nbld = min(160, max(4, int(round(t / 1250))))
+        mb, bounds, boxes = city_block(seed=seed * 1000 + i, n_buildings=nbld, target_tris=t,
+                                       nbands=nb)
+        specs.append((mb, nb))
+        geo.append((bounds, boxes))
+    queries = []
+    for j in range(n_queries):
+        si = rng.randint(0, n_scenes - 1)
+        n_src = int(round(rng.log_uniform(*sources)))
+        n_rays = int(round(rng.log_uniform(*rays)))
+        n_lis = rng.randint(*listeners)
+        bounds, boxes = geo[si]
+        base = seed * 100000 + j * 10
+        k = n_src + n_lis
+        mix = np.empty((2 * k, 3), np.float32)
+        mix[0::2] = indoor_points(base + 1, k, boxes)
+        mix[1::2] = outdoor_points(base + 2, k, bounds, boxes)
+        src, lis = mix[:n_src], mix[n_src:k]
+        queries.append(dict(scene=si, n_src=n_src, n_rays=n_rays, nbands=specs[si][1],
+                            max_time=float(rng.randint(*seconds)), seed=base,
+                            sources=np.ascontiguousarray(src, np.float32),
+                            listeners=np.ascontiguousarray(lis, np.float32)))
+    return specs, queries
