@@ -209,10 +209,10 @@ typedef struct {
 } omg_query;
 
 /* Launch shape of the persistent kernel: lanes per ray (1, or 4 = a DPP
- * quad), traversal-stack entries per ray kept in LDS (16 or 32), resident
- * waves per SIMD the grid is sized for (1..8), and regen = 1 to refill
- * lanes whose ray has ended with new rays between bounces (lanes = 1
- * only).  ms: the sampled time the autotuner measured for it (0 when set
+ * quad), traversal-stack entries per ray kept in LDS (16 or 32; 8, 12 or
+ * 16 with regen), resident waves per SIMD the grid is sized for (1..8),
+ * and regen = 1 to refill lanes whose ray has ended with new rays between
+ * bounces (lanes = 1 only).  ms: the sampled time the autotuner measured for it (0 when set
  * by hand). */
 typedef struct {
     uint32_t lanes;
@@ -376,6 +376,85 @@ int omg_solve_batch(omg_scene *scene, const omg_surface *surfaces, uint32_t n_su
                     const uint8_t *orders, uint32_t n_chains, const float listener[3],
                     const omg_blocker *extras, uint32_t n_extras, const float *air_per_m,
                     omg_mesh_record *records);
+
+/* ------------------------------------------------------------ serving loop */
+
+/* A propagation request (no reference counterpart: the reference traces
+ * one listener's late field per sim tick, late.rs:81-91, and samples the
+ * dome per tick, dome.rs:425-478).  Host memory, copied at submit. */
+typedef struct {
+    omg_scene *scene;           /* one of the server's resident scenes */
+    omg_trace_params params;
+    const float *sources;       /* [n_src][3] */
+    uint32_t n_src;
+    const float *listeners;     /* [n_lis][3] */
+    uint32_t n_lis;
+    uint32_t dome;              /* 1: also sample the dome fan at every listener */
+    uint32_t flags;             /* OMG_REQ_* */
+} omg_request;
+
+#define OMG_REQ_KEEP 1u         /* keep the results for omg_server_result */
+#define OMG_REQ_RECORDS 2u      /* also keep per-ray records (with OMG_REQ_KEEP) */
+
+typedef struct {
+    uint64_t hbm_budget;        /* bytes: resident scenes + in-flight work arenas */
+    uint32_t slots;             /* batches in flight at once, each on its own stream, 1..8 */
+    uint32_t max_batch;         /* requests per batch, 1..4096 */
+    omg_batch_shape shape;      /* persistent launch shape of every batch */
+    omg_dome_params dome;       /* dome fan of requests with dome = 1 */
+} omg_server_config;
+
+typedef struct {
+    uint64_t submitted, completed, refused, batches;
+    uint64_t committed_bytes;   /* resident scenes + work arenas now */
+    uint64_t peak_committed_bytes;
+    uint64_t rays;              /* tracer rays launched so far */
+    double mean_batch;          /* requests per batch */
+    uint32_t max_in_flight;     /* most batches on the device at once */
+} omg_server_stats;
+
+typedef struct {
+    double queue_s;             /* submit -> launch */
+    double latency_s;           /* arrival (submit, or the replay's scheduled time) -> results on the host */
+    uint32_t batch_size;        /* requests in the batch it ran in */
+    uint32_t slot;
+} omg_request_stats;
+
+typedef struct omg_server omg_server;
+
+/* Starts the scheduler thread over resident scenes.  The scenes' device
+ * bytes count against the budget (OMG_ERR_BUDGET if they alone exceed
+ * it).  While the server runs, its thread is the only user of the
+ * context's device work; the caller's thread only submits and collects. */
+int omg_server_create(omg_ctx *ctx, const omg_server_config *cfg, omg_scene *const *scenes,
+                      uint32_t n_scenes, omg_server **out);
+/* Queues a request; *ticket identifies it.  Refused with OMG_ERR_BUDGET
+ * when its work footprint could never fit beside the resident scenes.
+ * Requests are admitted to a batch in arrival order as a slot and budget
+ * free up; batches launch as soon as a slot is free (continuous batching:
+ * requests arriving while others run go into the next batch, which
+ * overlaps them on the device). */
+int omg_server_submit(omg_server *srv, const omg_request *req, uint64_t *ticket);
+/* Results of a finished OMG_REQ_KEEP request, laid out as omg_trace_world's
+ * (bins, dirs, records) and omg_dome_sample's (dome_energy [n_lis][9][nbands],
+ * dome_dsum [n_lis][9][3]); any output may be NULL.  OMG_ERR_NOT_READY while
+ * it runs.  The results are released after a successful call. */
+int omg_server_result(omg_server *srv, uint64_t ticket, float *bins, float *dirs,
+                      omg_ray_record *records, float *dome_energy, float *dome_dsum,
+                      omg_request_stats *st);
+/* Blocks until the request has finished (then as omg_server_result). */
+int omg_server_wait(omg_server *srv, uint64_t ticket, float *bins, float *dirs,
+                    omg_ray_record *records, float *dome_energy, float *dome_dsum,
+                    omg_request_stats *st);
+/* Open-loop load: submits reqs[i] at arrival_s[i] seconds after the call
+ * (ascending), waits for all, and returns each request's latency from its
+ * scheduled arrival to its results on the host, and its wait for a launch
+ * (queue_s, may be NULL).  A refused request reads -1.  tickets may be NULL. */
+int omg_server_replay(omg_server *srv, const omg_request *reqs, const double *arrival_s,
+                      uint32_t n, double *latency_s, double *queue_s, uint64_t *tickets);
+int omg_server_stats_get(omg_server *srv, omg_server_stats *st);
+/* Drains the in-flight work, stops the thread, frees the arenas. */
+void omg_server_destroy(omg_server *srv);
 
 #ifdef __cplusplus
 }

@@ -116,14 +116,14 @@ void batch_free(omg_batch* b)
     b->stream = nullptr;
 }
 
-bool valid_shape(const BatchShape& s)
-{
-    return (s.lanes == 1 || s.lanes == 4) && (s.stack_lds == 16 || s.stack_lds == 32) &&
-           s.waves_per_simd >= 1 && s.waves_per_simd <= 8 && s.regen <= 1 &&
-           !(s.regen && s.lanes != 1);
-}
-
 }  // namespace
+
+bool omg::valid_batch_shape(const BatchShape& s)
+{
+    if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 1) return false;
+    if (s.regen) return s.lanes == 1 && (s.stack_lds == 8 || s.stack_lds == 12 || s.stack_lds == 16);
+    return (s.lanes == 1 || s.lanes == 4) && (s.stack_lds == 16 || s.stack_lds == 32);
+}
 
 extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n, uint32_t flags,
                                 omg_batch** out)
@@ -358,9 +358,10 @@ extern "C" int omg_batch_set_shape(omg_batch* b, const omg_batch_shape* s)
 {
     if (!b || !s) return fail(OMG_ERR_INVALID, "bad argument");
     BatchShape sh{s->lanes, s->stack_lds, s->waves_per_simd, s->regen};
-    if (!valid_shape(sh))
+    if (!valid_batch_shape(sh))
         return fail(OMG_ERR_INVALID,
-                    "lanes 1|4, stack_lds 16|32, waves_per_simd 1..8, regen 0|1 (lanes 1 only)");
+                    "shape: lanes 1|4 with stack_lds 16|32, or regen 1 with lanes 1 and "
+                    "stack_lds 8|12|16; waves_per_simd 1..8");
     OMG_HIP_CHECK(hipSetDevice(b->ctx->device));
     int rc = omg_batch_sync(b);
     if (rc) return rc;
@@ -393,29 +394,32 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
     BatchShape best = b->shape;
     float best_ms = 1e30f;
     hipError_t e = hipSuccess;
-    // lanes x regen: one lane per ray with and without regeneration, or a quad per ray
-    const uint32_t modes[3][2] = {{1u, 1u}, {1u, 0u}, {4u, 0u}};
-    for (const auto& m : modes)
-        for (uint32_t sk : {16u, 32u})
-            for (uint32_t wps : {2u, 4u, 6u, 8u}) {
-                BatchShape sh{m[0], sk, wps, m[1]};
-                float ms = 1e30f;
-                for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {
-                    e = hipEventRecord(e0, b->stream);
-                    if (e == hipSuccess) e = launch_fill_zero_u64(b->ctl.as<unsigned long long>(), 1, b->stream);
-                    if (e == hipSuccess) e = launch_trace_batch(B, b->nbm, sh, c->n_cu, false, b->stream);
-                    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-                    if (e == hipSuccess) e = hipEventSynchronize(e1);
-                    float t = 0.0f;
-                    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-                    if (e == hipSuccess) ms = std::min(ms, t);
-                }
-                if (e != hipSuccess) break;
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = sh;
-                }
+    // (lanes, stack_lds, regen): one lane per ray with regeneration, one
+    // lane or a quad per ray without; each at 2..8 resident waves per SIMD
+    const uint32_t modes[7][3] = {{1u, 8u, 1u}, {1u, 12u, 1u}, {1u, 16u, 1u}, {1u, 16u, 0u},
+                                  {1u, 32u, 0u}, {4u, 16u, 0u}, {4u, 32u, 0u}};
+    for (const auto& m : modes) {
+        for (uint32_t wps : {2u, 4u, 6u, 8u}) {
+            BatchShape sh{m[0], m[1], wps, m[2]};
+            float ms = 1e30f;
+            for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {
+                e = hipEventRecord(e0, b->stream);
+                if (e == hipSuccess) e = launch_fill_zero_u64(b->ctl.as<unsigned long long>(), 1, b->stream);
+                if (e == hipSuccess) e = launch_trace_batch(B, b->nbm, sh, c->n_cu, false, b->stream);
+                if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float t = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+                if (e == hipSuccess) ms = std::min(ms, t);
             }
+            if (e != hipSuccess) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = sh;
+            }
+        }
+        if (e != hipSuccess) break;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (e != hipSuccess) return fail(OMG_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));

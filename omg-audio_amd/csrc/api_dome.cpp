@@ -16,11 +16,9 @@ struct omg_dome_probe {
     bool primed = false;
 };
 
-namespace {
-
 // DomeProbe::new fan (dome.rs:316-324) with the project sin/cos, and the
 // sector of each direction (bin_of, dome.rs:304-310).
-void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins)
+void omg::make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins)
 {
     fan.resize((size_t)n_fan * 3);
     bins.resize(n_fan);
@@ -42,6 +40,8 @@ void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bi
         }
     }
 }
+
+namespace {
 
 int validate(const omg_scene* s, const omg_dome_params* p, uint32_t n_eyes, uint32_t n_panels)
 {

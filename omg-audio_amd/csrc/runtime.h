@@ -39,6 +39,17 @@ struct DevBuf {
         if (e == hipSuccess) cap = want;
         return e;
     }
+    // grows to exactly `bytes` (no slack): for arenas whose size is budgeted
+    hipError_t ensure_exact(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&ptr, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
     void release()
     {
         if (ptr) (void)hipFree(ptr);
@@ -104,6 +115,8 @@ int validate_params(const omg_trace_params* p, uint32_t nbands, uint32_t n_src, 
 void job_tables(const std::vector<MatParams>& base_mats, const omg_overlay* ovl, uint32_t n_ovl,
                 uint32_t nbands, std::vector<MatParams>& mats, std::vector<Overlay>& ov);
 void job_scalars(const omg_trace_params* p, uint32_t nbands, uint32_t n_ovl, JobView& A);
-// drops one scene/batch reference; tears the context down after omg_shutdown
+// drops one scene/batch/server reference; tears the context down after omg_shutdown
 void ctx_release(omg_ctx* c);
+void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins);
+bool valid_batch_shape(const BatchShape& s);
 }  // namespace omg

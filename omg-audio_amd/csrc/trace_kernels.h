@@ -71,7 +71,7 @@ struct BatchArgs {
 // Launch shape of the batched kernel (chosen by the autotuner).
 struct BatchShape {
     uint32_t lanes;           // lanes per ray: 1 or 4
-    uint32_t stack_lds;       // 16 or 32 stack entries per ray in LDS
+    uint32_t stack_lds;       // stack entries per ray in LDS: 16 or 32; with regen 8, 12 or 16
     uint32_t waves_per_simd;  // resident waves per SIMD the grid is sized for
     uint32_t regen;           // 1: dead lanes take new rays between bounces (lanes = 1 only)
 };

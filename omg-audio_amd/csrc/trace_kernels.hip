@@ -594,9 +594,14 @@ template <int NBM, bool COUNT>
 static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint32_t blocks,
                                   hipStream_t s)
 {
-    if (sh.regen)
-        return sh.stack_lds >= 32 ? launch_batch_regen<NBM, COUNT, 32>(B, blocks, s)
-                                  : launch_batch_regen<NBM, COUNT, 16>(B, blocks, s);
+    if (sh.regen) {
+        switch (sh.stack_lds) {
+        case 8: return launch_batch_regen<NBM, COUNT, 8>(B, blocks, s);
+        case 12: return launch_batch_regen<NBM, COUNT, 12>(B, blocks, s);
+        case 16: return launch_batch_regen<NBM, COUNT, 16>(B, blocks, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (sh.stack_lds >= 32)
         return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 32, 4>(B, blocks, s)
                              : launch_batch_l<NBM, COUNT, 32, 1>(B, blocks, s);

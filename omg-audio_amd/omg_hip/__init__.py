@@ -386,3 +386,6 @@ __all__ += ["DomeProbe", "dome_params", "dome_sample", "dome_sample_async", "dom
 from .batch import Batch, QuerySpec  # noqa: E402
 
 __all__ += ["Batch", "QuerySpec"]
+from .serve import Server  # noqa: E402
+
+__all__ += ["Server"]

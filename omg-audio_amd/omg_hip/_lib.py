@@ -105,6 +105,33 @@ BATCH_RECORDS = 1
 BATCH_COUNTERS = 2
 
 
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", TraceParams), ("sources", C.c_void_p),
+                ("n_src", C.c_uint32), ("listeners", C.c_void_p), ("n_lis", C.c_uint32),
+                ("dome", C.c_uint32), ("flags", C.c_uint32)]
+
+
+REQ_KEEP = 1
+REQ_RECORDS = 2
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("max_batch", C.c_uint32),
+                ("shape", BatchShape), ("dome", DomeParams)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("refused", C.c_uint64),
+                ("batches", C.c_uint64), ("committed_bytes", C.c_uint64),
+                ("peak_committed_bytes", C.c_uint64), ("rays", C.c_uint64),
+                ("mean_batch", C.c_double), ("max_in_flight", C.c_uint32)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("queue_s", C.c_double), ("latency_s", C.c_double), ("batch_size", C.c_uint32),
+                ("slot", C.c_uint32)]
+
+
 # (name, restype, argtypes) of every exported entry point
 _vp, _u32, _f32, _i = C.c_void_p, C.c_uint32, C.c_float, C.c_int
 SIGNATURES = [
@@ -140,6 +167,13 @@ SIGNATURES = [
     ("omg_batch_set_shape", _i, [_vp, _vp]),
     ("omg_batch_tune", _i, [_vp, _u32, _vp]),
     ("omg_batch_destroy", None, [_vp]),
+    ("omg_server_create", _i, [_vp, _vp, _vp, _u32, C.POINTER(_vp)]),
+    ("omg_server_submit", _i, [_vp, _vp, C.POINTER(C.c_uint64)]),
+    ("omg_server_result", _i, [_vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("omg_server_wait", _i, [_vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("omg_server_replay", _i, [_vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    ("omg_server_stats_get", _i, [_vp, _vp]),
+    ("omg_server_destroy", None, [_vp]),
     ("omg_estimate_reverb", _i, [_vp, _u32, _u32, _f32, _vp, _vp]),
     ("omg_late_direction", _i, [_vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp]),
     ("omg_dome_sample", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),

@@ -320,3 +320,67 @@ def c3_workload(seed=3, n_queries=256, n_scenes=32, tris=(1_000, 200_000), sourc
                             sources=np.ascontiguousarray(src, np.float32),
                             listeners=np.ascontiguousarray(lis, np.float32)))
     return specs, queries
+
+
+def c5_workload(seed=5, n_scenes=8, tris=(5_000, 500_000), n_walkers=1000, qps=2000.0,
+                seconds=2.0, sources=(1, 32), rays=(512, 16384), speed=1.4, nbands=3):
+    """Serving workload: n_scenes resident city blocks (triangle targets
+    log-U[tris]) and a Poisson request stream at `qps` for `seconds`.
+
+    Listeners are n_walkers walkers, walker w living in scene w % n_scenes,
+    starting at a street point and walking at `speed` m/s with a heading
+    that turns by up to +-0.5 rad per request, reflected at the block's
+    bounds.  A request takes a uniformly drawn walker at its arrival time,
+    round(log-U[sources]) sources drawn from a per-scene pool of indoor and
+    street points, and round(log-U[rays]) rays per source.  Returns
+    (scene_specs [(builder, bounds, boxes)], requests [dict(scene, t,
+    n_src, n_rays, sources, listener, seed)]).
+    """
+    rng = SplitMix64(seed)
+    specs = []
+    for i in range(n_scenes):
+        t = int(round(rng.log_uniform(*tris)))
+        nbld = min(400, max(4, int(round(t / 1250))))
+        mb, bounds, boxes = city_block(seed=seed * 1000 + i, n_buildings=nbld, target_tris=t,
+                                       nbands=nbands)
+        specs.append((mb, bounds, boxes))
+    pools = []
+    for i, (_, bounds, boxes) in enumerate(specs):
+        pts = np.empty((512, 3), np.float32)
+        pts[0::2] = indoor_points(seed * 7000 + i, 256, boxes)
+        pts[1::2] = outdoor_points(seed * 7100 + i, 256, bounds, boxes)
+        pools.append(pts)
+    walkers = []
+    for w in range(n_walkers):
+        si = w % n_scenes
+        _, bounds, boxes = specs[si]
+        p = outdoor_points(seed * 100000 + w, 1, bounds, boxes)[0].astype(np.float64)
+        walkers.append([si, p, rng.uniform(0.0, 2.0 * math.pi), 0.0])
+    reqs = []
+    t = 0.0
+    while True:
+        t += -math.log(max(rng.uniform(), 1e-300)) / qps
+        if t >= seconds:
+            break
+        w = walkers[rng.randint(0, n_walkers - 1)]
+        si, p, heading, t_last = w
+        lo, hi = specs[si][1]
+        step = speed * (t - t_last)
+        p[0] += step * math.cos(heading)
+        p[1] += step * math.sin(heading)
+        for k in range(2):
+            a, b = float(lo[k]) + 1.0, float(hi[k]) - 1.0
+            if p[k] < a:
+                p[k] = 2 * a - p[k]
+            if p[k] > b:
+                p[k] = 2 * b - p[k]
+        w[2] = heading + rng.uniform(-0.5, 0.5)
+        w[3] = t
+        n_src = int(round(rng.log_uniform(*sources)))
+        n_rays = int(round(rng.log_uniform(*rays)))
+        pool = pools[si]
+        idx = [rng.randint(0, len(pool) - 1) for _ in range(n_src)]
+        reqs.append(dict(scene=si, t=t, n_src=n_src, n_rays=n_rays,
+                         sources=np.ascontiguousarray(pool[idx]),
+                         listener=p.astype(np.float32).reshape(1, 3), seed=seed * 1000003 + len(reqs)))
+    return specs, reqs

@@ -123,7 +123,7 @@ def test_batch_query_against_oracle(ctx, pool):
 
 @pytest.mark.parametrize("lanes,stack,wps,regen", [(1, 16, 2, 0), (1, 16, 8, 0), (4, 16, 4, 0),
                                                    (1, 32, 4, 0), (4, 32, 6, 0), (1, 16, 4, 1),
-                                                   (1, 16, 8, 1), (1, 32, 2, 1)])
+                                                   (1, 16, 8, 1), (1, 8, 2, 1), (1, 12, 6, 1)])
 def test_batch_shapes_and_replays(ctx, pool, lanes, stack, wps, regen):
     qs = _queries(pool)
     _say(f"case batch shape lanes={lanes} stack_lds={stack} waves/simd={wps} regen={regen}")
@@ -177,4 +177,6 @@ def test_batch_refusals(ctx, pool):
         b.set_shape(2, 16, 4)
     with pytest.raises(H.OmgError):
         b.set_shape(4, 16, 4, regen=1)   # regeneration runs one lane per ray
+    with pytest.raises(H.OmgError):
+        b.set_shape(1, 32, 4, regen=1)   # and keeps at most 16 stack entries in LDS
     b.close()
