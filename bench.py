@@ -23,6 +23,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "omg-audio_amd"))
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+# HBM bytes per k_trace launch of this workload, measured with rocprofv3 PMC
+# (FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, scales from a 256 MiB copy
+# kernel), profiles/k_trace_city_block_50k.md
+K_TRACE_TRAFFIC_BYTES = 19.12e6
+# the same for the tuned k_trace_batch_regen launch of the mixed 256-query
+# workload, profiles/k_trace_batch_regen_mixed_256q.md
+BATCH_TRAFFIC_BYTES = 87.72e9
 NODE_BYTES = 32           # bytes per AABB test (one child box, crates/omg-gpu/src/layout.rs:96-107)
 PRIM_BYTES = 48           # bytes per triangle test (layout.rs:109-122)
 
@@ -34,6 +41,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, run a plain elementwise copy of 256 MiB "
+                         "(read 256 MiB, write 256 MiB): the calibration kernel of HBM "
+                         "byte counters")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="minimum wall time of the CPU baseline sample")
     return ap.parse_args()
@@ -121,12 +133,59 @@ def batched_vs_plain(ctx, launches=3):
         "batched": {"s": batched_s, "bounce_rays_per_s": br / batched_s,
                     "queries_per_s": len(Q) / batched_s, "launches": launches,
                     "shape": shape, "tune_s": tune_s,
-                    "roofline_frac": alg / batched_s / 1e9 / HBM_PEAK_GBS},
+                    "roofline_frac": alg / batched_s / 1e9 / HBM_PEAK_GBS,
+                    "alg_bytes_per_launch": alg, "traffic": BATCH_TRAFFIC_BYTES},
         "plain": {"s": plain_s, "bounce_rays_per_s": br / plain_s,
                   "queries_per_s": len(Q) / plain_s},
         "speedup": plain_s / batched_s,
         "query0_rel_diff_vs_plain": rel,
     }
+
+
+def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
+    """Serving loop on 8 resident city blocks (5k-500k tris): Poisson
+    arrivals of requests from 1000 walking listeners, each 1-32 sources x
+    512-16384 rays (10 ms x 3 s echograms, 3 bands) plus the listener's
+    512-ray dome fan, 64 GB HBM budget.  One second of arrivals is replayed
+    open-loop at each rate; latency runs from the scheduled arrival to the
+    results on the host."""
+    import numpy as np
+    import omg_hip as H
+    from omg_hip import scenes
+    specs, reqs = scenes.c5_workload(seconds=2.0, qps=2000.0)
+    pool = [H.Scene.from_builder(ctx, mb, nbands=3) for mb, _, _ in specs]
+    R = [dict(scene=pool[r["scene"]], params=H.trace_params(r["n_rays"], 3, seed=r["seed"]),
+              sources=r["sources"], listeners=r["listener"], dome=True) for r in reqs]
+    arr0 = np.array([r["t"] for r in reqs])
+    shape = dict(lanes=1, stack_lds=8, waves_per_simd=8, regen=1)
+    srv = H.Server(ctx, pool, hbm_budget=64 << 30, slots=2, max_batch=64, shape=shape)
+    srv.replay(R[:100], arr0[:100])
+    rows = []
+    for qps in qps_list:
+        n = min(len(R), int(qps * seconds))
+        arr = arr0[:n] * (2000.0 / qps)
+        t0 = time.perf_counter()
+        lat, qs, _ = srv.replay(R[:n], arr, queue=True)
+        wall = time.perf_counter() - t0
+        rays = sum(R[i]["params"].n_rays * len(R[i]["sources"]) for i in range(n))
+        rows.append({"offered_qps": qps, "requests": n, "served_qps": n / wall,
+                     "rays_per_s": rays / wall,
+                     "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "queue_p99_ms": float(np.percentile(qs, 99) * 1e3)})
+    st = srv.stats()
+    srv.close()
+    for s in pool:
+        s.close()
+    held = [r["offered_qps"] for r in rows if r["p99_ms"] < 10.0]
+    return {"workload": "8 resident city blocks "
+                        f"({min(mb.n_tris for mb, _, _ in specs)}-{max(mb.n_tris for mb, _, _ in specs)} tris), "
+                        "Poisson arrivals, 1000 walkers at 1.4 m/s, 1-32 sources x 512-16384 rays "
+                        "+ 512-ray dome per request, 64 GB budget",
+            "shape": shape, "slots": 2, "rates": rows,
+            "max_qps_with_p99_under_10ms": max(held) if held else None,
+            "batches": st["batches"], "mean_batch": st["mean_batch"],
+            "peak_committed_bytes": st["peak_committed_bytes"]}
 
 
 def main():
@@ -244,13 +303,22 @@ def main():
                    "aabb_tests": tc["aabb_tests"], "tri_tests": tc["tri_tests"]},
         "dome": {"ray_events_per_step": dome_events, "ms": dome_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": K_TRACE_TRAFFIC_BYTES,
                      "kernel": "k_trace", "alg_bytes_per_launch": alg_bytes_trace},
         "wall_s": wall,
         "timed_vs_counted_rel_err": rel,
     }
+    if args.copy_probe:
+        x = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(2):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
     if not args.no_batch:
         out["batched_vs_plain"] = batched_vs_plain(ctx)
+    if not args.no_serve:
+        out["serving"] = serving(ctx)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb
