@@ -108,12 +108,19 @@ void omg_shutdown(omg_ctx *ctx);
 /* Thread-local message for the last failing call on this thread. */
 const char *omg_last_error(void);
 int omg_device_name(omg_ctx *ctx, char *buf, uint32_t len);
-/* Tracer launch shape (no reference counterpart; set by the autotuner):
- * BVH nodes of the top levels staged in LDS per block, traversal-stack
- * entries per ray kept in LDS (16 or 32; deeper entries spill to scratch),
- * and lanes per ray (1, 4 = a DPP quad per ray, 0 = by launch size).
- * Results do not depend on it. */
-int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds, uint32_t lanes);
+/* Tracer launch shape (no reference counterpart): BVH nodes of the top
+ * levels staged in LDS per block (in 64 B paired-node units; a 4-wide node
+ * takes two), traversal-stack entries per ray kept in LDS (16 or 32;
+ * deeper entries spill to scratch), lanes per ray (1, 4 = a DPP quad per
+ * ray, 0 = by launch size) and the BVH width walked (2 = the reference's
+ * binary tree, 4 = its 4-wide collapse; scenes whose 4-wide stack would
+ * exceed 64 entries keep width 2).  Width 2 reproduces the reference's
+ * traversal counts; both widths give the same nearest hits, except that
+ * among hits at exactly equal distance (flagged) the winner may differ.
+ * Defaults: 512, 16, 0, 4.  A call that asks for traversal counters walks
+ * width 2 (the counters are the reference traversal's). */
+int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds, uint32_t lanes,
+                          uint32_t width);
 
 /* ------------------------------------------------------------ scenes */
 
@@ -148,6 +155,16 @@ int omg_bvh_build_host(const float *positions, uint32_t nv, const uint32_t *indi
                        const float *tri_max_edge, float *prims_out, uint32_t prim_cap,
                        uint32_t *n_prims, float *nodes_out, uint32_t node_cap, uint32_t *n_nodes,
                        uint32_t *root_ref, uint32_t *max_depth);
+
+/* Host-only inspection of the 4-wide collapse of the same tree: nodes4_out
+ * rows are 32 words (per child: min[3], ref, max[3], 0; ref 0xFFFFFFFF =
+ * unused slot, bit 31 = leaf as above, else a row index); NULL queries the
+ * counts.  *stack4 is the worst-case traversal stack; *usable = 1 when it
+ * fits the 64-entry stack (the scene then walks width 4 on request). */
+int omg_bvh4_build_host(const float *positions, uint32_t nv, const uint32_t *indices, uint32_t nt,
+                        const uint16_t *tri_material, const uint16_t *tri_surface,
+                        const float *tri_max_edge, float *nodes4_out, uint32_t node_cap,
+                        uint32_t *n_nodes4, uint32_t *root4, uint32_t *stack4, uint32_t *usable);
 
 void omg_scene_destroy(omg_scene *scene);
 
@@ -219,11 +236,12 @@ typedef struct {
     uint32_t stack_lds;
     uint32_t waves_per_simd;
     uint32_t regen;
+    uint32_t width;             /* BVH width: 2, or 4 with regen = 1 when every scene has it */
     float ms;
 } omg_batch_shape;
 
 #define OMG_BATCH_RECORDS 1u   /* keep per-ray records (omg_batch_result) */
-#define OMG_BATCH_COUNTERS 2u  /* count traversal work (omg_batch_counters) */
+#define OMG_BATCH_COUNTERS 2u  /* count traversal work (omg_batch_counters); walks width 2 */
 
 typedef struct omg_batch omg_batch;
 

@@ -46,7 +46,8 @@ struct omg_batch {
     uint64_t rays = 0;
     uint32_t n_items = 0, nbm = 0;
     uint64_t class_key = 0;
-    BatchShape shape{1, 16, 4, 1};
+    BatchShape shape{1, 16, 4, 1, 2};
+    bool all4 = true;              // every scene has its 4-wide tree
     float shape_ms = 0.0f;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;
@@ -86,8 +87,11 @@ int capture(omg_batch* b)
     hipError_t e = launch_fill_zero_u64(b->ctl.as<unsigned long long>(), 4, b->stream);
     if (e == hipSuccess) e = launch_fill_zero(b->bins.as<float>(), b->n_bins_f, b->stream);
     if (e == hipSuccess) e = launch_fill_zero(b->dirs.as<float>(), b->n_dirs_f, b->stream);
+    // traversal counters are the reference walk's: a counting batch walks the binary tree
+    BatchShape sh = b->shape;
+    if (b->flags & OMG_BATCH_COUNTERS) sh.width = 2;
     if (e == hipSuccess)
-        e = launch_trace_batch(B, b->nbm, b->shape, c->n_cu, (b->flags & OMG_BATCH_COUNTERS) != 0,
+        e = launch_trace_batch(B, b->nbm, sh, c->n_cu, (b->flags & OMG_BATCH_COUNTERS) != 0,
                                b->stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
@@ -121,6 +125,7 @@ void batch_free(omg_batch* b)
 bool omg::valid_batch_shape(const BatchShape& s)
 {
     if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 1) return false;
+    if (s.width != 2 && !(s.width == 4 && s.regen)) return false;
     if (s.regen) return s.lanes == 1 && (s.stack_lds == 8 || s.stack_lds == 12 || s.stack_lds == 16);
     return (s.lanes == 1 || s.lanes == 4) && (s.stack_lds == 16 || s.stack_lds == 32);
 }
@@ -136,6 +141,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
     uint32_t nbm = 1;
     uint64_t rays = 0, items = 0;
     double lg_prims = 0.0;
+    bool all4 = true;
     for (uint32_t i = 0; i < n; i++) {
         const omg_query& Q = queries[i];
         if (!Q.scene || Q.scene->ctx != c || !Q.sources || !Q.listeners || (Q.n_ovl && !Q.overlays))
@@ -143,6 +149,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         int rc = validate_params(&Q.params, Q.scene->nbands, Q.n_src, Q.n_lis, Q.n_ovl, &nbins[i]);
         if (rc) return fail(rc, "query " + std::to_string(i) + ": " + omg_last_error());
         nbm = std::max(nbm, Q.scene->nbands);
+        all4 = all4 && Q.scene->has4;
         uint64_t r = (uint64_t)Q.params.n_rays * Q.n_src;
         rays += r;
         items += (uint64_t)Q.n_src * ((Q.params.n_rays + kBatchItemRays - 1) / kBatchItemRays);
@@ -156,6 +163,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
     b->flags = flags;
     b->nbm = nbm;
     b->rays = rays;
+    b->all4 = all4;
     b->q.resize(n);
     // host staging of every query's tables: mats | overlays | sources | listeners
     std::vector<size_t> off_mats(n), off_ovl(n), off_src(n), off_lis(n);
@@ -258,8 +266,10 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         uint32_t lg = rays ? (uint32_t)std::lround(lg_prims / (double)rays) : 0;
         b->class_key = ((uint64_t)(nbm <= 3 ? 3 : nbm <= 6 ? 6 : 8) << 16) |
                        ((uint64_t)log2_ceil(rays) << 8) | lg;
+        // untuned default: the regenerating walk (4-wide where every scene has it)
+        b->shape = all4 ? BatchShape{1, 8, 8, 1, 4} : BatchShape{1, 8, 8, 1, 2};
         auto it = c->tuned.find(b->class_key);
-        if (it != c->tuned.end()) b->shape = it->second;
+        if (it != c->tuned.end() && (it->second.width == 2 || all4)) b->shape = it->second;
         rc = capture(b);
     }
     if (rc) {
@@ -350,6 +360,7 @@ extern "C" int omg_batch_get_shape(omg_batch* b, omg_batch_shape* s)
     s->stack_lds = b->shape.stack_lds;
     s->waves_per_simd = b->shape.waves_per_simd;
     s->regen = b->shape.regen;
+    s->width = b->shape.width;
     s->ms = b->shape_ms;
     return OMG_OK;
 }
@@ -357,11 +368,12 @@ extern "C" int omg_batch_get_shape(omg_batch* b, omg_batch_shape* s)
 extern "C" int omg_batch_set_shape(omg_batch* b, const omg_batch_shape* s)
 {
     if (!b || !s) return fail(OMG_ERR_INVALID, "bad argument");
-    BatchShape sh{s->lanes, s->stack_lds, s->waves_per_simd, s->regen};
+    BatchShape sh{s->lanes, s->stack_lds, s->waves_per_simd, s->regen, s->width};
     if (!valid_batch_shape(sh))
         return fail(OMG_ERR_INVALID,
-                    "shape: lanes 1|4 with stack_lds 16|32, or regen 1 with lanes 1 and "
-                    "stack_lds 8|12|16; waves_per_simd 1..8");
+                    "shape: lanes 1|4 with stack_lds 16|32 and width 2, or regen 1 with lanes 1, "
+                    "stack_lds 8|12|16 and width 2|4; waves_per_simd 1..8");
+    if (sh.width == 4 && !b->all4) return fail(OMG_ERR_INVALID, "a scene of the batch has no 4-wide tree");
     OMG_HIP_CHECK(hipSetDevice(b->ctx->device));
     int rc = omg_batch_sync(b);
     if (rc) return rc;
@@ -394,13 +406,17 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
     BatchShape best = b->shape;
     float best_ms = 1e30f;
     hipError_t e = hipSuccess;
-    // (lanes, stack_lds, regen): one lane per ray with regeneration, one
-    // lane or a quad per ray without; each at 2..8 resident waves per SIMD
-    const uint32_t modes[7][3] = {{1u, 8u, 1u}, {1u, 12u, 1u}, {1u, 16u, 1u}, {1u, 16u, 0u},
-                                  {1u, 32u, 0u}, {4u, 16u, 0u}, {4u, 32u, 0u}};
+    // (lanes, stack_lds, regen, width): one lane per ray with regeneration
+    // over the binary or the 4-wide tree, one lane or a quad per ray
+    // without; each at 2..8 resident waves per SIMD
+    const uint32_t modes[10][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
+                                   {1u, 8u, 1u, 4u},  {1u, 12u, 1u, 4u}, {1u, 16u, 1u, 4u},
+                                   {1u, 16u, 0u, 2u}, {1u, 32u, 0u, 2u}, {4u, 16u, 0u, 2u},
+                                   {4u, 32u, 0u, 2u}};
     for (const auto& m : modes) {
+        if (m[3] == 4 && !b->all4) continue;
         for (uint32_t wps : {2u, 4u, 6u, 8u}) {
-            BatchShape sh{m[0], m[1], wps, m[2]};
+            BatchShape sh{m[0], m[1], wps, m[2], m[3]};
             float ms = 1e30f;
             for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {
                 e = hipEventRecord(e0, b->stream);

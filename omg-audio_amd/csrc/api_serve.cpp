@@ -460,8 +460,12 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
     *out = nullptr;
     if (cfg->slots < 1 || cfg->slots > 8) return fail(OMG_ERR_INVALID, "slots must be 1..8");
     if (cfg->max_batch < 1 || cfg->max_batch > 4096) return fail(OMG_ERR_INVALID, "max_batch must be 1..4096");
-    BatchShape sh{cfg->shape.lanes, cfg->shape.stack_lds, cfg->shape.waves_per_simd, cfg->shape.regen};
+    BatchShape sh{cfg->shape.lanes, cfg->shape.stack_lds, cfg->shape.waves_per_simd, cfg->shape.regen,
+                  cfg->shape.width};
     if (!valid_batch_shape(sh)) return fail(OMG_ERR_INVALID, "launch shape");
+    for (uint32_t i = 0; i < n_scenes; i++)
+        if (sh.width == 4 && scenes[i] && !scenes[i]->has4)
+            return fail(OMG_ERR_INVALID, "a scene has no 4-wide tree");
     const omg_dome_params& dp = cfg->dome;
     if (dp.n_fan == 0 || dp.n_rays == 0 || dp.n_rays > dp.n_fan || dp.max_events == 0 || dp.max_events > 64)
         return fail(OMG_ERR_INVALID, "dome parameters");

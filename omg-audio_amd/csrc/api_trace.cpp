@@ -1,4 +1,5 @@
 // C ABI: lifecycle, scenes, the late-field tracer and its post-processing.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -74,9 +75,10 @@ extern "C" void omg_shutdown(omg_ctx* c)
 }
 
 extern "C" int omg_set_launch_config(omg_ctx* c, uint32_t node_cache, uint32_t stack_lds,
-                                     uint32_t lanes)
+                                     uint32_t lanes, uint32_t width)
 {
     if (!c) return fail(OMG_ERR_INVALID, "bad argument");
+    if (width != 2 && width != 4) return fail(OMG_ERR_INVALID, "width must be 2 or 4");
     if (stack_lds != 16 && stack_lds != 32) return fail(OMG_ERR_INVALID, "stack_lds must be 16 or 32");
     if (lanes != 0 && lanes != 1 && lanes != 4) return fail(OMG_ERR_INVALID, "lanes must be 0, 1 or 4");
     size_t lds = (size_t)node_cache * sizeof(Node2) + (size_t)stack_lds * 256 * 8;
@@ -84,6 +86,7 @@ extern "C" int omg_set_launch_config(omg_ctx* c, uint32_t node_cache, uint32_t s
     c->node_cache = node_cache;
     c->stack_lds = stack_lds;
     c->lanes = lanes;
+    c->width = width;
     return OMG_OK;
 }
 
@@ -144,6 +147,10 @@ extern "C" int omg_scene_create(omg_ctx* c, const float* positions, uint32_t nv,
         s->root_max[k] = H.root_max[k];
     }
     s->empty = H.empty;
+    s->has4 = H.has4;
+    s->root4 = H.root4;
+    s->stack4 = H.stack4;
+    s->n_nodes4 = (uint32_t)H.nodes4.size();
     s->materials.assign(materials, materials + nm);
     s->mat_params.resize(nm);
     for (uint32_t m = 0; m < nm; m++)
@@ -162,6 +169,7 @@ extern "C" int omg_scene_create(omg_ctx* c, const float* positions, uint32_t nv,
     if (!rc) rc = upload(s->mats_raw, s->materials.data(), s->materials.size() * sizeof(omg_material), c->stream);
     std::vector<uint32_t> ts32(s->tri_surface.begin(), s->tri_surface.end());
     if (!rc) rc = upload(s->tri_surf, ts32.data(), ts32.size() * sizeof(uint32_t), c->stream);
+    if (!rc && s->has4) rc = upload(s->nodes4, H.nodes4.data(), H.nodes4.size() * sizeof(Node4), c->stream);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "scene upload");
     if (rc) {
         s->nodes.release();
@@ -170,11 +178,12 @@ extern "C" int omg_scene_create(omg_ctx* c, const float* positions, uint32_t nv,
         s->mats.release();
         s->mats_raw.release();
         s->tri_surf.release();
+        s->nodes4.release();
         delete s;
         return rc;
     }
     s->device_bytes = s->nodes.cap + s->prims.cap + s->tris.cap + s->mats.cap + s->mats_raw.cap +
-                      s->tri_surf.cap;
+                      s->tri_surf.cap + s->nodes4.cap;
     c->live_scenes++;
     *out = s;
     return OMG_OK;
@@ -246,6 +255,30 @@ extern "C" int omg_bvh_build_host(const float* positions, uint32_t nv, const uin
     return OMG_OK;
 }
 
+extern "C" int omg_bvh4_build_host(const float* positions, uint32_t nv, const uint32_t* indices,
+                                   uint32_t nt, const uint16_t* tri_material,
+                                   const uint16_t* tri_surface, const float* tri_max_edge,
+                                   float* nodes4_out, uint32_t node_cap, uint32_t* n_nodes4,
+                                   uint32_t* root4, uint32_t* stack4, uint32_t* usable)
+{
+    if ((nt && (!positions || !indices || !tri_material)) || !n_nodes4 || !root4 || !stack4 ||
+        !usable)
+        return fail(OMG_ERR_INVALID, "bad argument");
+    HostScene H;
+    std::string err;
+    if (build_scene(positions, nv, indices, nt, tri_material, tri_surface, tri_max_edge, H, err))
+        return fail(OMG_ERR_INVALID, err);
+    *n_nodes4 = (uint32_t)H.nodes4.size();
+    *root4 = H.root4;
+    *stack4 = H.stack4;
+    *usable = H.has4 ? 1u : 0u;
+    if (nodes4_out) {
+        if (node_cap < H.nodes4.size()) return fail(OMG_ERR_CAP, "node buffer too small");
+        std::memcpy(nodes4_out, H.nodes4.data(), H.nodes4.size() * sizeof(Node4));
+    }
+    return OMG_OK;
+}
+
 extern "C" void omg_scene_destroy(omg_scene* s)
 {
     if (!s) return;
@@ -257,6 +290,7 @@ extern "C" void omg_scene_destroy(omg_scene* s)
     s->mats.release();
     s->mats_raw.release();
     s->tri_surf.release();
+    s->nodes4.release();
     omg_ctx* c = s->ctx;
     delete s;
     ctx_release(c);
@@ -419,10 +453,11 @@ void omg::fill_geom(const omg_scene* s, GeomArgs& g)
     g.nodes = s->nodes.as<Node2>();
     g.prims = s->prims.as<Prim>();
     g.tris = s->tris.as<TriInfo>();
+    g.nodes4 = s->has4 ? s->nodes4.as<Node4>() : nullptr;
     g.root_ref = s->root_ref;
+    g.root4 = s->root4;
     g.empty = s->empty ? 1u : 0u;
     g.box[0] = g.box[1] = g.box[2] = 0.0f;
-    g.pad = 0;
     for (int k = 0; k < 3; k++) {
         g.root_min[k] = s->root_min[k];
         g.root_max[k] = s->root_max[k];
@@ -446,6 +481,9 @@ extern "C" int omg_trace_world(omg_scene* s, const omg_trace_params* p, const fl
     A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
     A.stack_lds = c->stack_lds;
     A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
+    // traversal counters are the reference walk's: counted on the binary tree
+    A.width = c->width == 4 && s->has4 && !counters ? 4u : 2u;
+    if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
@@ -473,6 +511,8 @@ extern "C" int omg_trace_world_hits(omg_scene* s, const omg_trace_params* p, con
     A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
     A.stack_lds = c->stack_lds;
     A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
+    A.width = c->width == 4 && s->has4 ? 4u : 2u;
+    if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
@@ -526,6 +566,8 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
     A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
     A.stack_lds = c->stack_lds;
     A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
+    A.width = c->width == 4 && s->has4 ? 4u : 2u;
+    if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
     // the job tables are staged on the context stream, then ordered before the launch
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;

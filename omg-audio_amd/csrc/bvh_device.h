@@ -421,6 +421,270 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
     return H.result(t_out, tri_out, flags);
 }
 
+// ------------------------------------------------------------------ 4-wide traversal
+//
+// The same tree collapsed into 128 B nodes of up to four children
+// (Node4).  At a node every child box is tested against the current best
+// distance; the hit children are ordered by entry distance (equal entries:
+// the tree's left-to-right order), the nearest becomes the current node and
+// the others are pushed farthest first.  Leaves are tested as in the binary
+// walk.  Every triangle with t < best lies in boxes entered at or before t,
+// so the nearest hit is the binary walk's; the order among hits at exactly
+// equal t can differ and such hits carry the tie flag.
+
+// Sort key of a child: entry distance, then slot.  A missed child sorts last.
+struct Cand4 {
+    float t;
+    uint32_t ref;
+    uint32_t slot;
+};
+
+// Field-wise selects: selecting whole structs would pick between two
+// addresses and push the array to scratch memory.
+__device__ __forceinline__ void cand_swap(Cand4& a, Cand4& b)
+{
+    bool sw = b.t < a.t || (b.t == a.t && b.slot < a.slot);
+    float ta = sw ? b.t : a.t, tb = sw ? a.t : b.t;
+    uint32_t ra = sw ? b.ref : a.ref, rb = sw ? a.ref : b.ref;
+    uint32_t sa = sw ? b.slot : a.slot, sb = sw ? a.slot : b.slot;
+    a.t = ta;
+    a.ref = ra;
+    a.slot = sa;
+    b.t = tb;
+    b.ref = rb;
+    b.slot = sb;
+}
+
+// 5-comparator network: c[0..3] ascending by (t, slot)
+__device__ __forceinline__ void sort4(Cand4 c[4])
+{
+    cand_swap(c[0], c[1]);
+    cand_swap(c[2], c[3]);
+    cand_swap(c[0], c[2]);
+    cand_swap(c[1], c[3]);
+    cand_swap(c[1], c[2]);
+}
+
+struct Lds4Trav {
+    const lds_float4* nodes;   // LDS node prefix [nc][8]
+    uint32_t nc;
+    lds_u32* st_ref;           // [SK][bs]
+    lds_f32* st_t;             // [SK][bs]
+    uint32_t lane, bs;
+};
+
+struct Quad4Trav {
+    const lds_float4* nodes;   // LDS node prefix [nc][8]
+    uint32_t nc;
+    lds_u32* st_ref;           // [SK][quads per block]
+    lds_f32* st_t;
+    uint32_t quad, nq, q;
+};
+
+// Tests the children of one 4-wide node already in registers (child j:
+// box lo[j].xyz..hi[j].xyz, ref lo[j].w) and orders them.
+__device__ __forceinline__ int order_children(const float4 lo[4], const float4 hi[4], f3 o, f3 inv,
+                                              float tmax, Cand4 c[4], uint32_t& tested)
+{
+    const float kMiss = __int_as_float(0x7f800000);
+    int nhit = 0;
+    tested = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint32_t ref = __float_as_uint(lo[j].w);
+        const float bmin[3] = {lo[j].x, lo[j].y, lo[j].z}, bmax[3] = {hi[j].x, hi[j].y, hi[j].z};
+        float te = 0.0f;
+        bool h = false;
+        if (ref != kNoChild) {
+            tested++;
+            h = ray_aabb(bmin, bmax, o, inv, tmax, te);
+        }
+        c[j].t = h ? te : kMiss;
+        c[j].ref = ref;
+        c[j].slot = (uint32_t)j;
+        nhit += h ? 1 : 0;
+    }
+    sort4(c);
+    return nhit;
+}
+
+// Nearest hit over the 4-wide nodes, one lane per ray, stack top in LDS.
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast4_lds(const GeomArgs& G, const Lds4Trav& L, f3 o, f3 d,
+                                                  float& t_out, uint32_t& tri_out, uint32_t& flags,
+                                                  TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    uint32_t ovf_ref[kStackCap - SK];
+    float ovf_t[kStackCap - SK];
+    int sp = 0;
+    uint32_t cur = G.root4;
+    float cur_t = 0.0f;
+    BestHit H;
+    auto push = [&](uint32_t ref, float t) {
+        if (sp < SK) {
+            L.st_ref[sp * L.bs + L.lane] = ref;
+            L.st_t[sp * L.bs + L.lane] = t;
+        } else {
+            ovf_ref[sp - SK] = ref;
+            ovf_t[sp - SK] = t;
+        }
+        sp++;
+    };
+    while (true) {
+        if (!(H.found() && cur_t >= H.best)) {
+            if (cur & kLeafBit) {
+                uint32_t start = cur & kLeafStartMask;
+                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+                for (uint32_t k = start; k < start + cnt_p; k++) {
+                    Prim p = load_prim(G.prims, k);
+                    if (COUNT) cnt.tri++;
+                    float t, u, v;
+                    if (ray_tri(p, o, d, t, u, v)) H.offer(t, p.tri, u, v);
+                }
+            } else {
+                float4 lo[4], hi[4];
+                if (cur < L.nc) {
+                    const lds_float4* q = L.nodes + 8 * cur;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        lo[j] = lds_ld4(q + 2 * j);
+                        hi[j] = lds_ld4(q + 2 * j + 1);
+                    }
+                } else {
+                    const float4* q = reinterpret_cast<const float4*>(G.nodes4 + cur);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        lo[j] = q[2 * j];
+                        hi[j] = q[2 * j + 1];
+                    }
+                }
+                Cand4 c[4];
+                uint32_t tested;
+                int nhit = order_children(lo, hi, o, inv, H.found() ? H.best : FLT_MAX, c, tested);
+                if (COUNT) cnt.aabb += tested;
+                if (nhit > 0) {
+                    if (nhit > 3) push(c[3].ref, c[3].t);
+                    if (nhit > 2) push(c[2].ref, c[2].t);
+                    if (nhit > 1) push(c[1].ref, c[1].t);
+                    cur = c[0].ref;
+                    cur_t = c[0].t;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        if (sp < SK) {
+            cur = L.st_ref[sp * L.bs + L.lane];
+            cur_t = L.st_t[sp * L.bs + L.lane];
+        } else {
+            cur = ovf_ref[sp - SK];
+            cur_t = ovf_t[sp - SK];
+        }
+    }
+    return H.result(t_out, tri_out, flags);
+}
+
+// The same walk on a DPP quad: lane q loads and tests child q (32 B) and
+// primitive q of a leaf; the four results are broadcast inside the quad and
+// every lane orders them identically.  Lane 0 writes the shared LDS stack.
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad4Trav& L, f3 o, f3 d,
+                                                   float& t_out, uint32_t& tri_out, uint32_t& flags,
+                                                   TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    uint32_t ovf_ref[kStackCap - SK];
+    float ovf_t[kStackCap - SK];
+    int sp = 0;
+    uint32_t cur = G.root4;
+    float cur_t = 0.0f;
+    BestHit H;
+    const uint32_t q = L.q;
+    const float kMiss = __int_as_float(0x7f800000);
+    auto push = [&](uint32_t ref, float t) {
+        if (sp < SK) {
+            if (q == 0) {
+                L.st_ref[sp * L.nq + L.quad] = ref;
+                L.st_t[sp * L.nq + L.quad] = t;
+            }
+        } else {
+            ovf_ref[sp - SK] = ref;
+            ovf_t[sp - SK] = t;
+        }
+        sp++;
+    };
+    while (true) {
+        if (!(H.found() && cur_t >= H.best)) {
+            if (cur & kLeafBit) {
+                uint32_t start = cur & kLeafStartMask;
+                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+                uint32_t hit_q = 0, tri_q = 0;
+                float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+                if (q < cnt_p) {
+                    Prim p = load_prim(G.prims, start + q);
+                    tri_q = p.tri;
+                    hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+                }
+                if (COUNT && q == 0) cnt.tri += cnt_p;
+                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+            } else {
+                float4 lo, hi;
+                if (cur < L.nc) {
+                    const lds_float4* src = L.nodes + 8 * cur + 2 * q;
+                    lo = lds_ld4(src);
+                    hi = lds_ld4(src + 1);
+                } else {
+                    const float4* src = reinterpret_cast<const float4*>(G.nodes4 + cur) + 2 * q;
+                    lo = src[0];
+                    hi = src[1];
+                }
+                uint32_t ref = __float_as_uint(lo.w);
+                const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
+                float te = 0.0f;
+                bool h = ref != kNoChild &&
+                         ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te);
+                float tk = h ? te : kMiss;
+                Cand4 c[4];
+                c[0] = Cand4{qbf<0>(tk), qb<0>(ref), 0u};
+                c[1] = Cand4{qbf<1>(tk), qb<1>(ref), 1u};
+                c[2] = Cand4{qbf<2>(tk), qb<2>(ref), 2u};
+                c[3] = Cand4{qbf<3>(tk), qb<3>(ref), 3u};
+                if (COUNT && q == 0)
+                    cnt.aabb += (c[0].ref != kNoChild) + (c[1].ref != kNoChild) +
+                                (c[2].ref != kNoChild) + (c[3].ref != kNoChild);
+                int nhit = (c[0].t != kMiss) + (c[1].t != kMiss) + (c[2].t != kMiss) +
+                           (c[3].t != kMiss);
+                sort4(c);
+                if (nhit > 0) {
+                    if (nhit > 3) push(c[3].ref, c[3].t);
+                    if (nhit > 2) push(c[2].ref, c[2].t);
+                    if (nhit > 1) push(c[1].ref, c[1].t);
+                    cur = c[0].ref;
+                    cur_t = c[0].t;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        if (sp < SK) {
+            cur = L.st_ref[sp * L.nq + L.quad];
+            cur_t = L.st_t[sp * L.nq + L.quad];
+        } else {
+            cur = ovf_ref[sp - SK];
+            cur_t = ovf_t[sp - SK];
+        }
+    }
+    return H.result(t_out, tri_out, flags);
+}
+
 // One crossing of a segment: position along it (0..1), original triangle,
 // authored surface and material.
 struct SegHit {

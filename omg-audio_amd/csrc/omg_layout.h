@@ -34,6 +34,27 @@ struct alignas(64) Node2 {
 };
 static_assert(sizeof(Node2) == 64, "Node2 is one 64 B line");
 
+// 4-wide node: the same tree collapsed so that one 128 B fetch serves up
+// to four child boxes (each internal child of a binary node is replaced by
+// its two children, largest surface area first, until four).  Children
+// keep the binary tree's left-to-right order; an unused slot has
+// ref == kNoChild.  Traversing it visits children nearest first, so the
+// nearest hit is the binary walk's; only the order among exactly equal
+// hit distances can differ, and such hits are flagged.
+constexpr uint32_t kNoChild = 0xFFFFFFFFu;
+
+struct alignas(16) Child4 {
+    float mn[3];
+    uint32_t ref;
+    float mx[3];
+    uint32_t pad;
+};
+
+struct alignas(128) Node4 {
+    Child4 c[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 is one 128 B line");
+
 // Packed primitive (mesh.rs:43-50): vertex a, edges e1 = b - a, e2 = c - a,
 // the ORIGINAL triangle id and its material.  48 B = three 16 B loads.
 struct alignas(16) Prim {

@@ -81,6 +81,7 @@ struct omg_ctx {
     uint32_t node_cache = 512;
     uint32_t stack_lds = 16;
     uint32_t lanes = 0;                 // 0: pick per launch (pick_lanes)
+    uint32_t width = 4;                 // BVH width of the plain launch (4 where built, else 2)
     uint64_t quad_below = 1ull << 20;   // launches with fewer rays trace a quad per ray
     uint32_t n_cu = 0;                  // compute units (persistent launches size their grid to it)
     bool closing = false;
@@ -99,7 +100,9 @@ struct omg_scene {
     std::vector<uint32_t> indices;
     std::vector<uint16_t> tri_material;
     std::vector<omg::TriInfo> tris_host;
-    omg::DevBuf nodes, prims, tris, mats, mats_raw, tri_surf;
+    omg::DevBuf nodes, prims, tris, mats, mats_raw, tri_surf, nodes4;
+    uint32_t n_nodes4 = 0, root4 = 0, stack4 = 0;
+    bool has4 = false;
     uint64_t device_bytes = 0;
 };
 

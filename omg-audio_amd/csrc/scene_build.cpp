@@ -217,6 +217,8 @@ void mat_params(const float* absorption, float scattering, const float* transmis
     }
 }
 
+void collapse4(HostScene& S);
+
 int build_scene(const float* positions, uint32_t nv, const uint32_t* indices, uint32_t nt,
                 const uint16_t* tri_material, const uint16_t* tri_surface,
                 const float* tri_max_edge, HostScene& S, std::string& err)
@@ -333,7 +335,96 @@ int build_scene(const float* positions, uint32_t nv, const uint32_t* indices, ui
     S.root_ref = ref_of(0);
     S.root_min[0] = B.nodes[0].bmin.x; S.root_min[1] = B.nodes[0].bmin.y; S.root_min[2] = B.nodes[0].bmin.z;
     S.root_max[0] = B.nodes[0].bmax.x; S.root_max[1] = B.nodes[0].bmax.y; S.root_max[2] = B.nodes[0].bmax.z;
+    collapse4(S);
     return 0;
+}
+
+// Collapses the paired binary nodes into 4-wide nodes, breadth-first: an
+// internal child (the one of largest box surface area, first in order on
+// equal areas) is replaced by its two children until four slots are used
+// or only leaves are left.  stack4 is the worst-case traversal stack: the
+// largest sum of (children - 1) along a root-to-leaf path.
+void collapse4(HostScene& S)
+{
+    S.nodes4.clear();
+    S.root4 = S.root_ref;
+    S.stack4 = 0;
+    S.has4 = true;
+    if (S.empty || (S.root_ref & kLeafBit)) return;
+    struct Kid {
+        float mn[3], mx[3];
+        uint32_t ref;
+    };
+    auto area = [](const Kid& k) {
+        float dx = k.mx[0] - k.mn[0], dy = k.mx[1] - k.mn[1], dz = k.mx[2] - k.mn[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    auto left = [](const Node2& n) {
+        return Kid{{n.lmin[0], n.lmin[1], n.lmin[2]}, {n.lmax[0], n.lmax[1], n.lmax[2]}, n.lref};
+    };
+    auto right = [](const Node2& n) {
+        return Kid{{n.rmin[0], n.rmin[1], n.rmin[2]}, {n.rmax[0], n.rmax[1], n.rmax[2]}, n.rref};
+    };
+    std::vector<uint32_t> queue{S.root_ref};  // binary node of each 4-wide node
+    S.nodes4.reserve(S.nodes.size() / 2 + 1);
+    for (size_t qi = 0; qi < queue.size(); qi++) {
+        const Node2& n = S.nodes[queue[qi]];
+        Kid k[4];
+        int nk = 2;
+        k[0] = left(n);
+        k[1] = right(n);
+        while (nk < 4) {
+            int pick = -1;
+            float pa = -1.0f;
+            for (int j = 0; j < nk; j++) {
+                if (k[j].ref & kLeafBit) continue;
+                float a = area(k[j]);
+                if (a > pa) {
+                    pa = a;
+                    pick = j;
+                }
+            }
+            if (pick < 0) break;
+            const Node2& m = S.nodes[k[pick].ref];
+            for (int j = nk; j > pick + 1; j--) k[j] = k[j - 1];
+            k[pick] = left(m);
+            k[pick + 1] = right(m);
+            nk++;
+        }
+        Node4 o;
+        std::memset(&o, 0, sizeof o);
+        for (int j = 0; j < 4; j++) {
+            Child4& c = o.c[j];
+            if (j >= nk) {
+                c.ref = kNoChild;
+                continue;
+            }
+            for (int a = 0; a < 3; a++) {
+                c.mn[a] = k[j].mn[a];
+                c.mx[a] = k[j].mx[a];
+            }
+            if (k[j].ref & kLeafBit) {
+                c.ref = k[j].ref;
+            } else {
+                c.ref = (uint32_t)queue.size();
+                queue.push_back(k[j].ref);
+            }
+        }
+        S.nodes4.push_back(o);
+    }
+    S.root4 = 0;
+    std::vector<uint32_t> need(S.nodes4.size(), 0);
+    for (size_t i = S.nodes4.size(); i-- > 0;) {
+        uint32_t nk = 0, deeper = 0;
+        for (const Child4& c : S.nodes4[i].c) {
+            if (c.ref == kNoChild) continue;
+            nk++;
+            if (!(c.ref & kLeafBit)) deeper = std::max(deeper, need[c.ref]);
+        }
+        need[i] = (nk - 1) + deeper;
+    }
+    S.stack4 = need[0];
+    S.has4 = S.stack4 + 1 <= (uint32_t)kStackCap;
 }
 
 }  // namespace omg

@@ -17,6 +17,11 @@ struct HostScene {
     uint32_t max_depth = 0;
     uint32_t n_bin_nodes = 0;
     bool empty = false;
+    // 4-wide collapse of the same tree (breadth-first); has4 when its
+    // worst-case traversal stack fits kStackCap
+    std::vector<Node4> nodes4;
+    uint32_t root4 = 0, stack4 = 0;
+    bool has4 = false;
 };
 
 // MeshBuilder::vertex dedup over a triangle soup; returns the vertex count.

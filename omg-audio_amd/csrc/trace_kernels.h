@@ -10,10 +10,11 @@ struct GeomArgs {
     const Node2* nodes;
     const Prim* prims;
     const TriInfo* tris;
+    const Node4* nodes4;     // 4-wide collapse of the same tree, or null
     uint32_t root_ref;
+    uint32_t root4;          // root of nodes4 (a leaf ref for one-leaf trees)
     uint32_t empty;          // mesh without primitives: every ray escapes
     float box[3];            // Shoebox size (box launches)
-    uint32_t pad;
     float root_min[3];       // bounds of the whole tree (the root's own box)
     float root_max[3];
 };
@@ -45,6 +46,7 @@ struct TraceArgs : JobView {
     uint32_t node_cache;     // BVH nodes (top levels) staged in LDS per block
     uint32_t stack_lds;      // traversal stack entries per ray kept in LDS (16 or 32)
     uint32_t lanes;          // lanes per ray: 1, or 4 (a DPP quad)
+    uint32_t width;          // BVH width walked: 2 (paired nodes) or 4 (nodes4)
     uint32_t* hit_ids;       // [rays][max_bounces] hit id per bounce (0xFFFFFFFF none), or null
     float* hit_t;            // [rays][max_bounces] hit distance per bounce, or null
 };
@@ -74,6 +76,7 @@ struct BatchShape {
     uint32_t stack_lds;       // stack entries per ray in LDS: 16 or 32; with regen 8, 12 or 16
     uint32_t waves_per_simd;  // resident waves per SIMD the grid is sized for
     uint32_t regen;           // 1: dead lanes take new rays between bounces (lanes = 1 only)
+    uint32_t width;           // BVH width walked: 2 or 4 (every scene of the batch has nodes4)
 };
 
 hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s);

@@ -98,6 +98,21 @@ __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const QuadTr
     return mesh_raycast_quad<COUNT, SK>(G, L, o, d, t, tri, fl, c);
 }
 
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Lds4Trav& L, f3 o, f3 d,
+                                                 float& t, uint32_t& tri, uint32_t& fl,
+                                                 TravCount& c)
+{
+    return mesh_raycast4_lds<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+}
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Quad4Trav& L, f3 o, f3 d,
+                                                 float& t, uint32_t& tri, uint32_t& fl,
+                                                 TravCount& c)
+{
+    return mesh_raycast4_quad<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+}
+
 template <bool MESH, bool COUNT, int SK, class Trav>
 __device__ __forceinline__ bool raycast_hit(const JobView& A, const Trav& L, f3 p, f3 d,
                                             Hit& h, TravCount& c)
@@ -124,33 +139,64 @@ __device__ __forceinline__ bool raycast_hit(const JobView& A, const Trav& L, f3 
     return found;
 }
 
-template <int LANES>
+// Traversal state type by lanes per ray and BVH width.
+template <int LANES, int W>
 struct TravOf {
     using type = LdsTrav;
 };
 template <>
-struct TravOf<4> {
+struct TravOf<4, 2> {
     using type = QuadTrav;
 };
+template <>
+struct TravOf<1, 4> {
+    using type = Lds4Trav;
+};
+template <>
+struct TravOf<4, 4> {
+    using type = Quad4Trav;
+};
 
-__device__ __forceinline__ void init_trav(LdsTrav& L, lds_float4* lds, uint32_t nc, int sk)
+// float4s per node of width W (64 B paired node, 128 B 4-wide node)
+template <int W>
+constexpr uint32_t node_f4() { return W == 4 ? 8u : 4u; }
+
+template <class T>
+__device__ __forceinline__ void init_lane_trav(T& L, lds_float4* lds, uint32_t nc, uint32_t nf4, int sk)
 {
     L.nc = nc;
     L.nodes = lds;
-    L.st_ref = reinterpret_cast<lds_u32*>(lds + 4 * nc);
+    L.st_ref = reinterpret_cast<lds_u32*>(lds + nf4 * nc);
     L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * blockDim.x);
     L.lane = threadIdx.x;
     L.bs = blockDim.x;
 }
-__device__ __forceinline__ void init_trav(QuadTrav& L, lds_float4* lds, uint32_t nc, int sk)
+template <class T>
+__device__ __forceinline__ void init_quad_trav(T& L, lds_float4* lds, uint32_t nc, uint32_t nf4, int sk)
 {
     L.nc = nc;
     L.nodes = lds;
     L.nq = blockDim.x >> 2;
-    L.st_ref = reinterpret_cast<lds_u32*>(lds + 4 * nc);
+    L.st_ref = reinterpret_cast<lds_u32*>(lds + nf4 * nc);
     L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nq);
     L.quad = threadIdx.x >> 2;
     L.q = threadIdx.x & 3u;
+}
+__device__ __forceinline__ void init_trav(LdsTrav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    init_lane_trav(L, lds, nc, 4u, sk);
+}
+__device__ __forceinline__ void init_trav(Lds4Trav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    init_lane_trav(L, lds, nc, 8u, sk);
+}
+__device__ __forceinline__ void init_trav(QuadTrav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    init_quad_trav(L, lds, nc, 4u, sk);
+}
+__device__ __forceinline__ void init_trav(Quad4Trav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    init_quad_trav(L, lds, nc, 8u, sk);
 }
 
 // State of one ray between bounces (tracer.rs:92-102).
@@ -337,15 +383,17 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uin
 // round-robin over the 8 XCDs; the block index is remapped so that each
 // XCD walks a contiguous range of rays (the same sources, hence the same
 // parts of the tree in its own L2).
-template <int NBM, bool MESH, bool COUNT, int SK, int LANES>
+template <int NBM, bool MESH, bool COUNT, int SK, int LANES, int W>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 {
     extern __shared__ float4 lds[];
-    typename TravOf<LANES>::type L;
+    typename TravOf<LANES, W>::type L;
     init_trav(L, (lds_float4*)lds, MESH ? A.node_cache : 0u, SK);
     if (MESH) {
-        const float4* src4 = reinterpret_cast<const float4*>(A.g.nodes);
-        for (uint32_t i = threadIdx.x; i < 4 * A.node_cache; i += blockDim.x) lds[i] = src4[i];
+        const float4* src4 = W == 4 ? reinterpret_cast<const float4*>(A.g.nodes4)
+                                    : reinterpret_cast<const float4*>(A.g.nodes);
+        for (uint32_t i = threadIdx.x; i < node_f4<W>() * A.node_cache; i += blockDim.x)
+            lds[i] = src4[i];
         __syncthreads();
     }
     uint32_t nb = gridDim.x, xcd = blockIdx.x & 7u, per = nb >> 3;
@@ -381,7 +429,7 @@ template <int NBM, bool COUNT, int SK, int LANES>
 __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs B)
 {
     extern __shared__ float4 lds[];
-    typename TravOf<LANES>::type L;
+    typename TravOf<LANES, 2>::type L;
     init_trav(L, (lds_float4*)lds, 0u, SK);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t q = LANES == 1 ? 0u : (threadIdx.x & (LANES - 1));
@@ -419,11 +467,16 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs B)
 // given consecutive rays; the wave takes a new item from the cursor when
 // its item is used up.  Lanes of one wave may then hold rays of different
 // queries: the job view is read per lane.  One lane per ray.
-template <int NBM, bool COUNT, int SK>
-__global__ __launch_bounds__(256) void k_trace_batch_regen(BatchArgs B)
+// Register budget: with 8 stack entries in LDS the kernel is held to 64
+// VGPRs (8 waves per SIMD), otherwise to 6 waves per SIMD; the few values
+// that no longer fit spill to scratch, which costs less than the lost
+// occupancy of this latency-bound walk.
+template <int NBM, bool COUNT, int SK, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SK <= 8 ? 8 : 6)))
+void k_trace_batch_regen(BatchArgs B)
 {
     extern __shared__ float4 lds[];
-    LdsTrav L;
+    typename TravOf<1, W>::type L;
     init_trav(L, (lds_float4*)lds, 0u, SK);
     const uint32_t lane = threadIdx.x & 63u;
     RayState<NBM> S;
@@ -497,15 +550,15 @@ __global__ void k_fill_zero_u64(unsigned long long* p, size_t n)
 
 constexpr uint32_t kTraceBlock = 256;
 
-template <int NBM, bool MESH, bool COUNT, int SK, int LANES>
+template <int NBM, bool MESH, bool COUNT, int SK, int LANES, int W>
 static hipError_t launch_trace_l(const TraceArgs& A, hipStream_t s)
 {
     uint64_t threads = (uint64_t)A.ray_count * LANES;
     uint32_t blocks = (uint32_t)((threads + kTraceBlock - 1) / kTraceBlock);
     if (blocks == 0) return hipSuccess;
-    size_t lds = (MESH ? (size_t)A.node_cache * sizeof(Node2) : 0) +
+    size_t lds = (MESH ? (size_t)A.node_cache * 16 * node_f4<W>() : 0) +
                  (size_t)SK * (kTraceBlock / LANES) * 8;
-    auto kern = k_trace<NBM, MESH, COUNT, SK, LANES>;
+    auto kern = k_trace<NBM, MESH, COUNT, SK, LANES, W>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -518,14 +571,26 @@ static hipError_t launch_trace_l(const TraceArgs& A, hipStream_t s)
 template <int NBM, bool MESH, bool COUNT, int SK>
 static hipError_t launch_trace_t(const TraceArgs& A, hipStream_t s)
 {
-    if (MESH && A.lanes == 4) return launch_trace_l<NBM, MESH, COUNT, SK, 4>(A, s);
-    return launch_trace_l<NBM, MESH, COUNT, SK, 1>(A, s);
+    if (MESH && A.lanes == 4) return launch_trace_l<NBM, MESH, COUNT, SK, 4, 2>(A, s);
+    return launch_trace_l<NBM, MESH, COUNT, SK, 1, 2>(A, s);
+}
+
+// 4-wide walks keep 16 stack entries per ray in LDS
+template <int NBM, bool COUNT>
+static hipError_t launch_trace_w4(const TraceArgs& A, hipStream_t s)
+{
+    if (A.lanes == 4) return launch_trace_l<NBM, true, COUNT, 16, 4, 4>(A, s);
+    return launch_trace_l<NBM, true, COUNT, 16, 1, 4>(A, s);
 }
 
 template <int NBM, bool MESH, bool COUNT>
 static hipError_t launch_trace_sk(const TraceArgs& A, hipStream_t s)
 {
     if (!MESH) return launch_trace_t<NBM, MESH, COUNT, 1>(A, s);
+    if (A.width == 4) {
+        if (!A.g.nodes4) return hipErrorInvalidValue;
+        return launch_trace_w4<NBM, COUNT>(A, s);
+    }
     if (A.stack_lds >= 32) return launch_trace_t<NBM, MESH, COUNT, 32>(A, s);
     return launch_trace_t<NBM, MESH, COUNT, 16>(A, s);
 }
@@ -582,11 +647,16 @@ static hipError_t launch_batch_l(const BatchArgs& B, uint32_t blocks, hipStream_
 }
 
 template <int NBM, bool COUNT, int SK>
-static hipError_t launch_batch_regen(const BatchArgs& B, uint32_t blocks, hipStream_t s)
+static hipError_t launch_batch_regen(const BatchArgs& B, uint32_t width, uint32_t blocks,
+                                     hipStream_t s)
 {
     size_t lds = (size_t)SK * kTraceBlock * 8;
-    hipLaunchKernelGGL((k_trace_batch_regen<NBM, COUNT, SK>), dim3(blocks), dim3(kTraceBlock), lds,
-                       s, B);
+    if (width == 4)
+        hipLaunchKernelGGL((k_trace_batch_regen<NBM, COUNT, SK, 4>), dim3(blocks), dim3(kTraceBlock),
+                           lds, s, B);
+    else
+        hipLaunchKernelGGL((k_trace_batch_regen<NBM, COUNT, SK, 2>), dim3(blocks), dim3(kTraceBlock),
+                           lds, s, B);
     return hipGetLastError();
 }
 
@@ -596,9 +666,9 @@ static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint
 {
     if (sh.regen) {
         switch (sh.stack_lds) {
-        case 8: return launch_batch_regen<NBM, COUNT, 8>(B, blocks, s);
-        case 12: return launch_batch_regen<NBM, COUNT, 12>(B, blocks, s);
-        case 16: return launch_batch_regen<NBM, COUNT, 16>(B, blocks, s);
+        case 8: return launch_batch_regen<NBM, COUNT, 8>(B, sh.width, blocks, s);
+        case 12: return launch_batch_regen<NBM, COUNT, 12>(B, sh.width, blocks, s);
+        case 16: return launch_batch_regen<NBM, COUNT, 16>(B, sh.width, blocks, s);
         default: return hipErrorInvalidValue;
         }
     }
@@ -617,6 +687,7 @@ hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape
     if (B.n_items == 0) return hipSuccess;
     if (sh.lanes != 1 && sh.lanes != 4) return hipErrorInvalidValue;
     if (sh.regen && sh.lanes != 1) return hipErrorInvalidValue;
+    if (sh.width != 2 && !(sh.width == 4 && sh.regen)) return hipErrorInvalidValue;
     if (sh.waves_per_simd == 0 || sh.waves_per_simd > 8) return hipErrorInvalidValue;
     // 4 SIMDs per CU and 4 waves per 256-lane block: one block per CU per wave slot
     uint32_t blocks = n_cu * sh.waves_per_simd;

@@ -89,10 +89,11 @@ class Context:
         check(lib().omg_device_name(self.h, buf, 128))
         self.device_name = buf.value.decode()
 
-    def set_launch_config(self, node_cache=512, stack_lds=16, lanes=0):
-        """Tracer launch shape: top BVH nodes staged in LDS, LDS stack depth,
-        lanes per ray (1, 4, or 0 = chosen by launch size)."""
-        check(lib().omg_set_launch_config(self.h, node_cache, stack_lds, lanes))
+    def set_launch_config(self, node_cache=512, stack_lds=16, lanes=0, width=4):
+        """Tracer launch shape: top BVH nodes staged in LDS (64 B units), LDS
+        stack depth, lanes per ray (1, 4, or 0 = chosen by launch size), BVH
+        width (2 = the reference's binary tree, 4 = its 4-wide collapse)."""
+        check(lib().omg_set_launch_config(self.h, node_cache, stack_lds, lanes, width))
 
     def close(self):
         if getattr(self, "h", None):

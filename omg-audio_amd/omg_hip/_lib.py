@@ -98,7 +98,7 @@ class Query(C.Structure):
 
 class BatchShape(C.Structure):
     _fields_ = [("lanes", C.c_uint32), ("stack_lds", C.c_uint32), ("waves_per_simd", C.c_uint32),
-                ("regen", C.c_uint32), ("ms", C.c_float)]
+                ("regen", C.c_uint32), ("width", C.c_uint32), ("ms", C.c_float)]
 
 
 BATCH_RECORDS = 1
@@ -139,7 +139,7 @@ SIGNATURES = [
     ("omg_shutdown", None, [_vp]),
     ("omg_last_error", C.c_char_p, []),
     ("omg_device_name", _i, [_vp, C.c_char_p, _u32]),
-    ("omg_set_launch_config", _i, [_vp, _u32, _u32, _u32]),
+    ("omg_set_launch_config", _i, [_vp, _u32, _u32, _u32, _u32]),
     ("omg_scene_create", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32,
                               C.POINTER(_vp)]),
     ("omg_scene_create_soup", _i, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32,
@@ -149,6 +149,9 @@ SIGNATURES = [
     ("omg_bvh_build_host", _i, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32,
                                 C.POINTER(_u32), _vp, _u32, C.POINTER(_u32), C.POINTER(_u32),
                                 C.POINTER(_u32)]),
+    ("omg_bvh4_build_host", _i, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32,
+                                 C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32),
+                                 C.POINTER(_u32)]),
     ("omg_scene_destroy", None, [_vp]),
     ("omg_trace_world", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("omg_trace_world_hits", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp,

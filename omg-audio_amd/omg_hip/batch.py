@@ -97,10 +97,10 @@ class Batch:
         s = BatchShape()
         check(lib().omg_batch_get_shape(self.h, C.byref(s)))
         return dict(lanes=s.lanes, stack_lds=s.stack_lds, waves_per_simd=s.waves_per_simd,
-                    regen=s.regen, ms=s.ms)
+                    regen=s.regen, width=s.width, ms=s.ms)
 
-    def set_shape(self, lanes=1, stack_lds=16, waves_per_simd=4, regen=0):
-        s = BatchShape(lanes, stack_lds, waves_per_simd, regen, 0.0)
+    def set_shape(self, lanes=1, stack_lds=16, waves_per_simd=4, regen=0, width=2):
+        s = BatchShape(lanes, stack_lds, waves_per_simd, regen, width, 0.0)
         check(lib().omg_batch_set_shape(self.h, C.byref(s)))
 
     def tune(self, stride=16):
@@ -108,7 +108,7 @@ class Batch:
         s = BatchShape()
         check(lib().omg_batch_tune(self.h, stride, C.byref(s)))
         return dict(lanes=s.lanes, stack_lds=s.stack_lds, waves_per_simd=s.waves_per_simd,
-                    regen=s.regen, ms=s.ms)
+                    regen=s.regen, width=s.width, ms=s.ms)
 
     def close(self):
         if getattr(self, "h", None):

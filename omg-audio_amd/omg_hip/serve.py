@@ -43,9 +43,10 @@ class Server:
         cfg.hbm_budget = int(hbm_budget)
         cfg.slots = slots
         cfg.max_batch = max_batch
-        sh = dict(lanes=1, stack_lds=16, waves_per_simd=4, regen=1)
+        sh = dict(lanes=1, stack_lds=8, waves_per_simd=8, regen=1, width=4)
         sh.update(shape or {})
-        cfg.shape = BatchShape(sh["lanes"], sh["stack_lds"], sh["waves_per_simd"], sh["regen"], 0.0)
+        cfg.shape = BatchShape(sh["lanes"], sh["stack_lds"], sh["waves_per_simd"], sh["regen"],
+                               sh["width"], 0.0)
         cfg.dome = dome if dome is not None else dome_params(3, n_fan=512, horizon=500.0)
         arr = (C.c_void_p * max(1, len(self.scenes)))(*[s.h.value if hasattr(s.h, "value") else s.h
                                                          for s in self.scenes])

@@ -65,13 +65,23 @@ def _queries(pool):
     return out
 
 
-def _plain(q, counters=False):
+def _plain(q):
+    """The query through omg_trace_world; counting makes it walk the binary tree."""
     return H.trace_world(q.scene, q.params, q.sources, q.listeners, q.overlays, records=True,
-                         counters=counters)
+                         counters=True)
 
 
-def _same(a, b, label):
-    mism = np.nonzero(np.any(a.records != b.records, axis=1))[0]
+def _same(a, b, label, wide=False):
+    """a's rays equal b's.  With wide (a 4-wide walk against the binary one)
+    rays whose reference record carries a near-edge or exact-tie flag may
+    differ; echograms are then compared only when no ray did."""
+    diff = np.any(a.records != b.records, axis=1)
+    if wide:
+        flagged = (b.records[:, 3] & 3) != 0
+        assert not np.any(diff & ~flagged), f"{label}: unflagged rays differ"
+        if diff.any():
+            return 0.0, 0.0
+    mism = np.nonzero(diff)[0]
     assert mism.size == 0, f"{label}: {mism.size} rays differ, first {mism[:5]}"
     eb, ed = echogram_rel_errors(a.bins, a.dirs, b.bins.astype(np.float64),
                                  b.dirs.astype(np.float64))
@@ -82,7 +92,7 @@ def _same(a, b, label):
 def test_batch_matches_plain_per_query(ctx, pool):
     qs = _queries(pool)
     _say(f"case batch vs plain: {len(qs)} queries")
-    plain = [_plain(q, counters=True) for q in qs]
+    plain = [_plain(q) for q in qs]
     b = H.Batch(ctx, qs, records=True, counters=True)
     rays, items = b.size()
     assert rays == sum(q.params.n_rays * len(q.sources) for q in qs)
@@ -116,26 +126,32 @@ def test_batch_query_against_oracle(ctx, pool):
                            src_index=si, dir_band=1, bin_dt=0.001, max_time=q.params.max_time,
                            floor_rel=1e-6, energy=[1.0 + 0.1 * b for b in range(6)])
         o = O.trace(g, s, q.listeners, p, want_rays=True, want_f64=True)
-        assert np.array_equal(rec[si], o["recs"]), f"source {si}"
+        # the default batch walks the 4-wide tree: only near-edge / tie-flagged rays may differ
+        diff = np.any(rec[si] != o["recs"], axis=1)
+        assert not np.any(diff & ((o["recs"][:, 3] & 3) == 0)), f"source {si}"
+        if diff.any():
+            continue
         eb, ed = echogram_rel_errors(got.bins[si], got.dirs[si], o["bins64"], o["dirs64"], 1)
         assert eb <= ECHO_TOL and ed <= ECHO_TOL, (si, eb, ed)
 
 
-@pytest.mark.parametrize("lanes,stack,wps,regen", [(1, 16, 2, 0), (1, 16, 8, 0), (4, 16, 4, 0),
-                                                   (1, 32, 4, 0), (4, 32, 6, 0), (1, 16, 4, 1),
-                                                   (1, 16, 8, 1), (1, 8, 2, 1), (1, 12, 6, 1)])
-def test_batch_shapes_and_replays(ctx, pool, lanes, stack, wps, regen):
+@pytest.mark.parametrize("lanes,stack,wps,regen,width",
+                         [(1, 16, 2, 0, 2), (1, 16, 8, 0, 2), (4, 16, 4, 0, 2), (1, 32, 4, 0, 2),
+                          (4, 32, 6, 0, 2), (1, 16, 4, 1, 2), (1, 16, 8, 1, 2), (1, 8, 2, 1, 2),
+                          (1, 12, 6, 1, 2), (1, 8, 8, 1, 4), (1, 16, 4, 1, 4)])
+def test_batch_shapes_and_replays(ctx, pool, lanes, stack, wps, regen, width):
     qs = _queries(pool)
-    _say(f"case batch shape lanes={lanes} stack_lds={stack} waves/simd={wps} regen={regen}")
+    _say(f"case batch shape lanes={lanes} stack_lds={stack} waves/simd={wps} regen={regen} "
+         f"width={width}")
     ref = [_plain(q) for q in qs]
     b = H.Batch(ctx, qs, records=True)
-    b.set_shape(lanes, stack, wps, regen)
+    b.set_shape(lanes, stack, wps, regen, width)
     assert b.shape["lanes"] == lanes and b.shape["stack_lds"] == stack
-    assert b.shape["regen"] == regen
+    assert b.shape["regen"] == regen and b.shape["width"] == width
     for rep in range(2):
         b.launch()
         for i in range(len(qs)):
-            _same(b.result(i), ref[i], f"replay {rep} query {i}")
+            _same(b.result(i), ref[i], f"replay {rep} query {i}", wide=width == 4)
     b.close()
 
 
@@ -144,12 +160,13 @@ def test_batch_tune(ctx, pool):
     _say("case batch autotune")
     b = H.Batch(ctx, qs, records=True)
     shape = b.tune(stride=2)
-    assert shape["lanes"] in (1, 4) and shape["stack_lds"] in (16, 32)
+    assert shape["lanes"] in (1, 4) and shape["stack_lds"] in (8, 12, 16, 32)
+    assert shape["width"] in (2, 4)
     assert 1 <= shape["waves_per_simd"] <= 8 and shape["ms"] > 0
     assert shape["regen"] in (0, 1) and not (shape["regen"] and shape["lanes"] != 1)
     b.launch()
     for i, q in enumerate(qs):
-        _same(b.result(i), _plain(q), f"tuned query {i}")
+        _same(b.result(i), _plain(q), f"tuned query {i}", wide=shape["width"] == 4)
     # a second batch of the same class starts from the cached choice
     b2 = H.Batch(ctx, qs, records=True)
     s2 = b2.shape
@@ -179,4 +196,6 @@ def test_batch_refusals(ctx, pool):
         b.set_shape(4, 16, 4, regen=1)   # regeneration runs one lane per ray
     with pytest.raises(H.OmgError):
         b.set_shape(1, 32, 4, regen=1)   # and keeps at most 16 stack entries in LDS
+    with pytest.raises(H.OmgError):
+        b.set_shape(1, 16, 4, regen=0, width=4)   # the 4-wide walk runs regenerating
     b.close()

@@ -133,3 +133,88 @@ def test_host_postprocessing_matches_oracle(oracle):
     ref = O.image_source_taps((8, 6, 3), [(m.absorption, m.scattering, m.transmission)
                                           for m in walls], (2, 2, 1.5), (5, 4, 1.5), 3)
     assert taps.shape == (63, 8) and np.array_equal(taps, ref)
+
+
+def _host_build4(mb):
+    L = _lib.lib()
+    soup = mb.soup()
+    pos, idx = O.dedup_tris(soup["tri_verts"])
+    nt = idx.shape[0]
+    args = [pos.ctypes.data, pos.shape[0], idx.ctypes.data, nt, soup["tri_material"].ctypes.data,
+            soup["tri_surface"].ctypes.data, soup["tri_max_edge"].ctypes.data]
+    n4, root4, stack4, usable = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _lib.check(L.omg_bvh4_build_host(*args, None, 0, C.byref(n4), C.byref(root4),
+                                     C.byref(stack4), C.byref(usable)))
+    nodes4 = np.zeros((max(1, n4.value), 32), np.float32)
+    _lib.check(L.omg_bvh4_build_host(*args, nodes4.ctypes.data, n4.value, C.byref(n4),
+                                     C.byref(root4), C.byref(stack4), C.byref(usable)))
+    return nodes4[:n4.value], root4.value, stack4.value, usable.value
+
+
+@pytest.mark.parametrize("which", ["shoebox", "city", "soup"])
+def test_bvh4_collapse_covers_the_binary_tree(which):
+    """The 4-wide tree holds the binary tree's leaves once each, every leaf
+    child keeps the binary box of that leaf, every internal child's box is
+    the union of its node's children, 2-4 children per node in
+    breadth-first order, and the stack bound is the largest sum of
+    (children - 1) along a root-to-leaf path."""
+    if which == "shoebox":
+        mb = scenes.shoebox(seed=1)[0]
+    elif which == "city":
+        mb = scenes.city_block(seed=2, n_buildings=8, target_tris=6000)[0]
+    else:
+        rng = scenes.SplitMix64(7)
+        mb = scenes.MeshBuilder()
+        m = mb.material(scenes.MaterialDef([0.1] * 3, 0.1, [0.0] * 3))
+        for _ in range(200):
+            b = [rng.uniform(0, 20) for _ in range(3)]
+            mb.tri(tuple(b), tuple(x + rng.uniform(-3, 3) for x in b),
+                   tuple(x + rng.uniform(-3, 3) for x in b), m)
+    _, _, _, nodes, root, _ = _host_build(mb)
+    n4, root4, stack4, usable = _host_build4(mb)
+    nu = nodes.view(np.uint32)
+    # binary: leaf ref -> its box
+    leaf_box = {}
+    stack = [root]
+    while stack:
+        r = stack.pop()
+        if r & 0x80000000:
+            continue
+        for refc, lo, hi in ((nu[r, 3], 0, 4), (nu[r, 7], 8, 12)):
+            if refc & 0x80000000:
+                leaf_box[int(refc)] = (nodes[r, lo:lo + 3].copy(), nodes[r, hi:hi + 3].copy())
+            else:
+                stack.append(int(refc))
+    if root & 0x80000000:
+        assert root4 == root and n4.shape[0] == 0
+        return
+    assert root4 == 0
+    u4 = n4.view(np.uint32)
+    seen = []
+    need = {}
+
+    def walk(i):
+        kids = [j for j in range(4) if u4[i, 8 * j + 3] != 0xFFFFFFFF]
+        assert 2 <= len(kids) <= 4 and kids == list(range(len(kids)))
+        deeper = 0
+        for j in kids:
+            ref = int(u4[i, 8 * j + 3])
+            mn, mx = n4[i, 8 * j:8 * j + 3], n4[i, 8 * j + 4:8 * j + 7]
+            if ref & 0x80000000:
+                seen.append(ref)
+                bmn, bmx = leaf_box[ref]
+                assert np.array_equal(mn, bmn) and np.array_equal(mx, bmx)
+            else:
+                assert ref > i
+                ck = [c for c in range(4) if u4[ref, 8 * c + 3] != 0xFFFFFFFF]
+                cmn = np.min([n4[ref, 8 * c:8 * c + 3] for c in ck], axis=0)
+                cmx = np.max([n4[ref, 8 * c + 4:8 * c + 7] for c in ck], axis=0)
+                assert np.array_equal(mn, cmn) and np.array_equal(mx, cmx)
+                deeper = max(deeper, walk(ref))
+        need[i] = len(kids) - 1 + deeper
+        return need[i]
+
+    assert walk(0) == stack4
+    assert sorted(seen) == sorted(leaf_box)
+    assert len(need) == n4.shape[0]
+    assert usable == (1 if stack4 + 1 <= 64 else 0)

@@ -126,7 +126,7 @@ def test_launch_shapes_give_identical_rays(ctx, oracle, shape):
     kw = dict(n_rays=1000, nbands=3, seed=31, bin_dt=0.001, max_time=2.0)
     scene = H.Scene.from_builder(ctx, mb)
     try:
-        ctx.set_launch_config(nc, sk, lanes)
+        ctx.set_launch_config(nc, sk, lanes, 2)
         gpu = H.trace_world(scene, H.trace_params(**kw), src, lis, records=True, counters=True)
     finally:
         ctx.set_launch_config()
@@ -136,6 +136,50 @@ def test_launch_shapes_give_identical_rays(ctx, oracle, shape):
     assert gpu.counters["aabb_tests"] == sum(int(o["stats"][0]) for o in outs)
     assert gpu.counters["tri_tests"] == sum(int(o["stats"][1]) for o in outs)
     assert gpu.counters["bounce_rays"] == sum(int(o["stats"][2]) for o in outs)
+
+
+@pytest.mark.parametrize("scene_name", ["city", "c1"])
+@pytest.mark.parametrize("shape", [(0, 1), (256, 1), (0, 4), (256, 4)])
+def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape):
+    """The 4-wide collapse finds the same nearest hit at every bounce: per-ray
+    records and per-bounce hit ids / distances equal the oracle's for every
+    ray whose oracle record carries no near-edge or exact-tie flag (the only
+    rays whose winner may depend on visiting order)."""
+    nc, lanes = shape
+    if scene_name == "city":
+        mb, bounds, boxes = scenes.city_block(seed=5, n_buildings=8, target_tris=6000)
+        src = np.vstack([scenes.indoor_points(51, 2, boxes),
+                         scenes.outdoor_points(52, 1, bounds, boxes)])
+        lis = scenes.outdoor_points(54, 2, bounds, boxes)
+    else:
+        mb, s1, l1, _ = scenes.shoebox(seed=1)
+        src, lis = np.array([s1]), np.array([l1])
+    _say(f"case width 4 {scene_name} node_cache={nc} lanes={lanes}")
+    kw = dict(n_rays=1000, nbands=3, seed=41, bin_dt=0.001, max_time=2.0)
+    scene = H.Scene.from_builder(ctx, mb)
+    try:
+        ctx.set_launch_config(nc, 16, lanes, 4)
+        gpu, ids, ts = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)
+    finally:
+        ctx.set_launch_config()
+    try:
+        ctx.set_launch_config(512, 16, 0, 2)
+        ref, ids2, ts2 = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)
+    finally:
+        ctx.set_launch_config()
+    om = oracle_mesh_from_builder(mb)
+    outs = oracle_trace_sources(O.mesh_geom(om), src, lis, **kw)
+    orec = np.vstack([o["recs"] for o in outs])
+    assert np.array_equal(ref.records, orec)
+    flagged = (orec[:, 3] & 3) != 0
+    diff = np.any(gpu.records != orec, axis=1)
+    assert not np.any(diff & ~flagged), f"{np.sum(diff & ~flagged)} unflagged rays differ"
+    ok = ~flagged
+    assert np.array_equal(ids[ok], ids2[ok]) and np.array_equal(ts[ok], ts2[ok])
+    _say(f"  {ok.sum()} unflagged rays identical hit by hit; {diff.sum()} of "
+         f"{flagged.sum()} flagged rays took another path")
+    if not diff.any():
+        compare(gpu, outs, 1000, label=f"width 4 {scene_name}")
 
 
 def test_overlays_six_bands(ctx, oracle):
