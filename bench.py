@@ -158,7 +158,8 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
               sources=r["sources"], listeners=r["listener"], dome=True) for r in reqs]
     arr0 = np.array([r["t"] for r in reqs])
     shape = dict(lanes=1, stack_lds=8, waves_per_simd=8, regen=1, width=4)
-    srv = H.Server(ctx, pool, hbm_budget=64 << 30, slots=2, max_batch=64, shape=shape)
+    srv = H.Server(ctx, pool, hbm_budget=64 << 30, slots=2, max_batch=64, shape=shape,
+                   latency_rays=1 << 20)
     srv.replay(R[:100], arr0[:100])
     rows = []
     for qps in qps_list:
@@ -182,7 +183,7 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
                         f"({min(mb.n_tris for mb, _, _ in specs)}-{max(mb.n_tris for mb, _, _ in specs)} tris), "
                         "Poisson arrivals, 1000 walkers at 1.4 m/s, 1-32 sources x 512-16384 rays "
                         "+ 512-ray dome per request, 64 GB budget",
-            "shape": shape, "slots": 2, "rates": rows,
+            "shape": shape, "slots": 2, "latency_rays": 1 << 20, "rates": rows,
             "max_qps_with_p99_under_10ms": max(held) if held else None,
             "batches": st["batches"], "mean_batch": st["mean_batch"],
             "peak_committed_bytes": st["peak_committed_bytes"]}

@@ -418,7 +418,9 @@ typedef struct {
     uint64_t hbm_budget;        /* bytes: resident scenes + in-flight work arenas */
     uint32_t slots;             /* batches in flight at once, each on its own stream, 1..8 */
     uint32_t max_batch;         /* requests per batch, 1..4096 */
-    omg_batch_shape shape;      /* persistent launch shape of every batch */
+    uint32_t latency_rays;      /* batches of at most this many rays run a DPP quad per ray
+                                   over 16-ray items (shorter per-ray chains); 0 = never */
+    omg_batch_shape shape;      /* persistent launch shape of the other batches */
     omg_dome_params dome;       /* dome fan of requests with dome = 1 */
 } omg_server_config;
 

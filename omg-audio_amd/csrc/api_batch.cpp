@@ -125,8 +125,9 @@ void batch_free(omg_batch* b)
 bool omg::valid_batch_shape(const BatchShape& s)
 {
     if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 1) return false;
-    if (s.width != 2 && !(s.width == 4 && s.regen)) return false;
+    if (s.width != 2 && s.width != 4) return false;
     if (s.regen) return s.lanes == 1 && (s.stack_lds == 8 || s.stack_lds == 12 || s.stack_lds == 16);
+    if (s.width == 4) return (s.lanes == 1 || s.lanes == 4) && s.stack_lds == 16;
     return (s.lanes == 1 || s.lanes == 4) && (s.stack_lds == 16 || s.stack_lds == 32);
 }
 
@@ -371,8 +372,8 @@ extern "C" int omg_batch_set_shape(omg_batch* b, const omg_batch_shape* s)
     BatchShape sh{s->lanes, s->stack_lds, s->waves_per_simd, s->regen, s->width};
     if (!valid_batch_shape(sh))
         return fail(OMG_ERR_INVALID,
-                    "shape: lanes 1|4 with stack_lds 16|32 and width 2, or regen 1 with lanes 1, "
-                    "stack_lds 8|12|16 and width 2|4; waves_per_simd 1..8");
+                    "shape: lanes 1|4 with stack_lds 16|32 (width 2) or 16 (width 4), or regen 1 "
+                    "with lanes 1, stack_lds 8|12|16 and width 2|4; waves_per_simd 1..8");
     if (sh.width == 4 && !b->all4) return fail(OMG_ERR_INVALID, "a scene of the batch has no 4-wide tree");
     OMG_HIP_CHECK(hipSetDevice(b->ctx->device));
     int rc = omg_batch_sync(b);
@@ -409,10 +410,10 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
     // (lanes, stack_lds, regen, width): one lane per ray with regeneration
     // over the binary or the 4-wide tree, one lane or a quad per ray
     // without; each at 2..8 resident waves per SIMD
-    const uint32_t modes[10][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
+    const uint32_t modes[12][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
                                    {1u, 8u, 1u, 4u},  {1u, 12u, 1u, 4u}, {1u, 16u, 1u, 4u},
                                    {1u, 16u, 0u, 2u}, {1u, 32u, 0u, 2u}, {4u, 16u, 0u, 2u},
-                                   {4u, 32u, 0u, 2u}};
+                                   {4u, 32u, 0u, 2u}, {1u, 16u, 0u, 4u}, {4u, 16u, 0u, 4u}};
     for (const auto& m : modes) {
         if (m[3] == 4 && !b->all4) continue;
         for (uint32_t wps : {2u, 4u, 6u, 8u}) {

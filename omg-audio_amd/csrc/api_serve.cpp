@@ -54,6 +54,9 @@ struct Req {
 };
 using ReqP = std::shared_ptr<Req>;
 
+// rays per work item of a latency-mode batch: one pass of a wave of quads
+constexpr uint32_t kLatencyItemRays = 16;
+
 struct Slot {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;    // dome fans, overlapped with the tracer
@@ -120,7 +123,7 @@ struct omg_server {
     {
         size_t b = a16(r.scene->mat_params.size() * sizeof(MatParams)) + a16(r.n_src * 12) +
                    a16(r.n_lis * 12) + a16(sizeof(JobView)) +
-                   a16((size_t)r.n_src * ((r.p.n_rays + kBatchItemRays - 1) / kBatchItemRays) *
+                   a16((size_t)r.n_src * ((r.p.n_rays + kLatencyItemRays - 1) / kLatencyItemRays) *
                        sizeof(BatchItem)) +
                    a16(r.n_bins_f() * 4) + a16(r.n_dirs_f() * 4);
         if (r.flags & OMG_REQ_RECORDS) b += a16((size_t)r.n_src * r.p.n_rays * sizeof(omg_ray_record));
@@ -163,6 +166,13 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
     for (uint32_t g = 0; g < ng; g++)
         for (uint32_t i : gmem[g]) g_eyes[g] += batch[i]->n_lis;
 
+    // small batches run in latency mode: a DPP quad per ray over 16-ray items
+    uint64_t rays = 0;
+    for (uint32_t i = 0; i < n; i++) rays += (uint64_t)batch[i]->n_src * batch[i]->p.n_rays;
+    const bool lat = cfg.latency_rays && rays <= cfg.latency_rays;
+    const uint32_t item_rays = lat ? kLatencyItemRays : kBatchItemRays;
+    const BatchShape sh = lat ? BatchShape{4, 16, 8, 0, shape.width} : shape;
+
     // inputs
     std::vector<size_t> o_mats(n), o_src(n), o_lis(n), g_eye_off(ng);
     size_t in = 0;
@@ -176,7 +186,7 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         in = a16(in + r.n_src * 12);
         o_lis[i] = in;
         in = a16(in + r.n_lis * 12);
-        items += (uint64_t)r.n_src * ((r.p.n_rays + kBatchItemRays - 1) / kBatchItemRays);
+        items += (uint64_t)r.n_src * ((r.p.n_rays + item_rays - 1) / item_rays);
         nbm = std::max(nbm, r.nbands);
     }
     for (uint32_t g = 0; g < ng; g++) {
@@ -261,8 +271,8 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         J.dirs = reinterpret_cast<float*>(D + r.o_dirs);
         J.records = (r.flags & OMG_REQ_RECORDS) ? reinterpret_cast<uint32_t*>(D + r.o_rec) : nullptr;
         for (uint32_t s = 0; s < r.n_src; s++)
-            for (uint32_t ray = 0; ray < r.p.n_rays; ray += kBatchItemRays)
-                it[k++] = BatchItem{i, s, ray, std::min(kBatchItemRays, r.p.n_rays - ray)};
+            for (uint32_t ray = 0; ray < r.p.n_rays; ray += item_rays)
+                it[k++] = BatchItem{i, s, ray, std::min(item_rays, r.p.n_rays - ray)};
     }
     for (uint32_t g = 0; g < ng; g++) {
         size_t e = 0;
@@ -284,7 +294,10 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         OMG_HIP_CHECK(hipEventRecord(S.fork, st));
         OMG_HIP_CHECK(hipStreamWaitEvent(S.side, S.fork, 0));
     }
-    OMG_HIP_CHECK(launch_trace_batch(B, nbm, shape, ctx->n_cu, false, st));
+    // a grid no larger than the batch needs leaves the rest of the chip to the other slots
+    uint64_t lanes_needed = lat ? items * 64 : rays;
+    uint32_t max_blocks = (uint32_t)std::min<uint64_t>((lanes_needed + 255) / 256, 1u << 20);
+    OMG_HIP_CHECK(launch_trace_batch(B, nbm, sh, ctx->n_cu, false, st, max_blocks));
     for (uint32_t g = 0; g < ng; g++) {
         DomeArgs A{};
         fill_geom(gscene[g], A.g);

@@ -76,13 +76,14 @@ struct BatchShape {
     uint32_t stack_lds;       // stack entries per ray in LDS: 16 or 32; with regen 8, 12 or 16
     uint32_t waves_per_simd;  // resident waves per SIMD the grid is sized for
     uint32_t regen;           // 1: dead lanes take new rays between bounces (lanes = 1 only)
-    uint32_t width;           // BVH width walked: 2 or 4 (every scene of the batch has nodes4)
+    uint32_t width;           // BVH width walked: 2 or 4 (every scene of the batch has nodes4;
+                              // without regen, width 4 keeps 16 stack entries)
 };
 
 hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s);
 // persistent work-stealing launch over n_cu compute units; nbm = the largest
 // band count of the batch
 hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape& shape,
-                              uint32_t n_cu, bool count, hipStream_t s);
+                              uint32_t n_cu, bool count, hipStream_t s, uint32_t max_blocks = 0);
 
 }  // namespace omg

@@ -425,11 +425,11 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 // record are bit-identical to the query's own omg_trace_world.  No BVH
 // nodes are staged in LDS (a wave moves between scenes); the LDS holds the
 // SK-deep stack tops only.
-template <int NBM, bool COUNT, int SK, int LANES>
+template <int NBM, bool COUNT, int SK, int LANES, int W>
 __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs B)
 {
     extern __shared__ float4 lds[];
-    typename TravOf<LANES, 2>::type L;
+    typename TravOf<LANES, W>::type L;
     init_trav(L, (lds_float4*)lds, 0u, SK);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t q = LANES == 1 ? 0u : (threadIdx.x & (LANES - 1));
@@ -637,12 +637,12 @@ hipError_t launch_fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s)
 
 namespace omg {
 
-template <int NBM, bool COUNT, int SK, int LANES>
+template <int NBM, bool COUNT, int SK, int LANES, int W>
 static hipError_t launch_batch_l(const BatchArgs& B, uint32_t blocks, hipStream_t s)
 {
     size_t lds = (size_t)SK * (kTraceBlock / LANES) * 8;
-    hipLaunchKernelGGL((k_trace_batch<NBM, COUNT, SK, LANES>), dim3(blocks), dim3(kTraceBlock), lds,
-                       s, B);
+    hipLaunchKernelGGL((k_trace_batch<NBM, COUNT, SK, LANES, W>), dim3(blocks), dim3(kTraceBlock),
+                       lds, s, B);
     return hipGetLastError();
 }
 
@@ -672,25 +672,31 @@ static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint
         default: return hipErrorInvalidValue;
         }
     }
+    if (sh.width == 4)
+        return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 16, 4, 4>(B, blocks, s)
+                             : launch_batch_l<NBM, COUNT, 16, 1, 4>(B, blocks, s);
     if (sh.stack_lds >= 32)
-        return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 32, 4>(B, blocks, s)
-                             : launch_batch_l<NBM, COUNT, 32, 1>(B, blocks, s);
-    return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 16, 4>(B, blocks, s)
-                         : launch_batch_l<NBM, COUNT, 16, 1>(B, blocks, s);
+        return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 32, 4, 2>(B, blocks, s)
+                             : launch_batch_l<NBM, COUNT, 32, 1, 2>(B, blocks, s);
+    return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 16, 4, 2>(B, blocks, s)
+                         : launch_batch_l<NBM, COUNT, 16, 1, 2>(B, blocks, s);
 }
 
 // Band capacities compiled for batches: 3, 6 and 8 (a batch runs at the
 // smallest one holding its widest query).
 hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape& sh,
-                              uint32_t n_cu, bool count, hipStream_t s)
+                              uint32_t n_cu, bool count, hipStream_t s, uint32_t max_blocks)
 {
     if (B.n_items == 0) return hipSuccess;
     if (sh.lanes != 1 && sh.lanes != 4) return hipErrorInvalidValue;
     if (sh.regen && sh.lanes != 1) return hipErrorInvalidValue;
-    if (sh.width != 2 && !(sh.width == 4 && sh.regen)) return hipErrorInvalidValue;
+    if (sh.width != 2 && sh.width != 4) return hipErrorInvalidValue;
+    if (sh.width == 4 && !sh.regen && sh.stack_lds != 16) return hipErrorInvalidValue;
     if (sh.waves_per_simd == 0 || sh.waves_per_simd > 8) return hipErrorInvalidValue;
-    // 4 SIMDs per CU and 4 waves per 256-lane block: one block per CU per wave slot
+    // 4 SIMDs per CU and 4 waves per 256-lane block: one block per CU per wave slot,
+    // fewer when the batch has less work than that (concurrent batches then co-run)
     uint32_t blocks = n_cu * sh.waves_per_simd;
+    if (max_blocks && max_blocks < blocks) blocks = max_blocks;
     if (nbm <= 3) return count ? launch_batch_nb<3, true>(B, sh, blocks, s) : launch_batch_nb<3, false>(B, sh, blocks, s);
     if (nbm <= 6) return count ? launch_batch_nb<6, true>(B, sh, blocks, s) : launch_batch_nb<6, false>(B, sh, blocks, s);
     if (nbm <= 8) return count ? launch_batch_nb<8, true>(B, sh, blocks, s) : launch_batch_nb<8, false>(B, sh, blocks, s);

@@ -117,7 +117,7 @@ REQ_RECORDS = 2
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("max_batch", C.c_uint32),
-                ("shape", BatchShape), ("dome", DomeParams)]
+                ("latency_rays", C.c_uint32), ("shape", BatchShape), ("dome", DomeParams)]
 
 
 class ServerStats(C.Structure):

@@ -36,13 +36,14 @@ class Server:
     """omg_server_*: create over resident scenes, submit, collect, replay, close."""
 
     def __init__(self, ctx, scenes, hbm_budget=64 << 30, slots=4, max_batch=64,
-                 shape=None, dome=None):
+                 shape=None, dome=None, latency_rays=0):
         self.ctx = ctx
         self.scenes = list(scenes)
         cfg = ServerConfig()
         cfg.hbm_budget = int(hbm_budget)
         cfg.slots = slots
         cfg.max_batch = max_batch
+        cfg.latency_rays = latency_rays
         sh = dict(lanes=1, stack_lds=8, waves_per_simd=8, regen=1, width=4)
         sh.update(shape or {})
         cfg.shape = BatchShape(sh["lanes"], sh["stack_lds"], sh["waves_per_simd"], sh["regen"],

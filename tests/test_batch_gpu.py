@@ -138,7 +138,8 @@ def test_batch_query_against_oracle(ctx, pool):
 @pytest.mark.parametrize("lanes,stack,wps,regen,width",
                          [(1, 16, 2, 0, 2), (1, 16, 8, 0, 2), (4, 16, 4, 0, 2), (1, 32, 4, 0, 2),
                           (4, 32, 6, 0, 2), (1, 16, 4, 1, 2), (1, 16, 8, 1, 2), (1, 8, 2, 1, 2),
-                          (1, 12, 6, 1, 2), (1, 8, 8, 1, 4), (1, 16, 4, 1, 4)])
+                          (1, 12, 6, 1, 2), (1, 8, 8, 1, 4), (1, 16, 4, 1, 4), (4, 16, 4, 0, 4),
+                          (1, 16, 6, 0, 4)])
 def test_batch_shapes_and_replays(ctx, pool, lanes, stack, wps, regen, width):
     qs = _queries(pool)
     _say(f"case batch shape lanes={lanes} stack_lds={stack} waves/simd={wps} regen={regen} "
@@ -197,5 +198,5 @@ def test_batch_refusals(ctx, pool):
     with pytest.raises(H.OmgError):
         b.set_shape(1, 32, 4, regen=1)   # and keeps at most 16 stack entries in LDS
     with pytest.raises(H.OmgError):
-        b.set_shape(1, 16, 4, regen=0, width=4)   # the 4-wide walk runs regenerating
+        b.set_shape(1, 32, 4, regen=0, width=4)   # the 4-wide item walk keeps 16 entries
     b.close()

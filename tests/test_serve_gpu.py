@@ -57,14 +57,18 @@ def _mix(world, n, rng):
     return out
 
 
-def test_request_alone_and_in_mix_match_plain(ctx, world):
+@pytest.mark.parametrize("latency_rays", [0, 1 << 20])
+def test_request_alone_and_in_mix_match_plain(ctx, world, latency_rays):
+    """latency_rays = 0: every batch runs the regenerating walk; 1 << 20:
+    these small batches run a DPP quad per ray over 16-ray items."""
     sa, sb = world["scenes"]
     pts = world["points"][1]
     probe = dict(scene=sb, params=_params(2500, 4242), sources=pts[[0, 3, 7]],
                  listeners=pts[[5, 9]], dome=True, keep=True, records=True)
     dp = H.dome_params(3, n_fan=512, horizon=500.0)
-    srv = H.Server(ctx, [sa, sb], hbm_budget=1 << 34, slots=3, max_batch=16, dome=dp)
-    _say("case serve: probe request alone")
+    srv = H.Server(ctx, [sa, sb], hbm_budget=1 << 34, slots=3, max_batch=16, dome=dp,
+                   latency_rays=latency_rays)
+    _say(f"case serve (latency_rays={latency_rays}): probe request alone")
     t = srv.submit(probe["scene"], probe["params"], probe["sources"], probe["listeners"],
                    dome=True, keep=True, records=True)
     alone = srv.wait(t)
