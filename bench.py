@@ -26,10 +26,11 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per k_trace launch of this workload, measured with rocprofv3 PMC
 # (FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, scales from a 256 MiB copy
 # kernel), profiles/k_trace_city_block_50k.md
-K_TRACE_TRAFFIC_BYTES = 19.12e6
-# the same for the tuned k_trace_batch_regen launch of the mixed 256-query
-# workload, profiles/k_trace_batch_regen_mixed_256q.md
-BATCH_TRAFFIC_BYTES = 87.72e9
+K_TRACE_TRAFFIC_BYTES = 18.44e6
+# the same for the k_trace_batch_regen launch of the mixed 256-query
+# workload in the shape the autotuner usually picks (1,8,8,1,4),
+# profiles/k_trace_batch_regen_mixed_256q.md
+BATCH_TRAFFIC_BYTES = 230.4e9
 NODE_BYTES = 32           # bytes per AABB test (one child box, crates/omg-gpu/src/layout.rs:96-107)
 PRIM_BYTES = 48           # bytes per triangle test (layout.rs:109-122)
 
@@ -42,6 +43,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--batch-shape", default="",
+                    help="lanes,stack_lds,waves_per_simd,regen,width for the batched launch "
+                         "instead of the autotuner's choice (reproducible profiles)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run a plain elementwise copy of 256 MiB "
                          "(read 256 MiB, write 256 MiB): the calibration kernel of HBM "
@@ -78,7 +82,7 @@ def cpu_baseline(mb, src, lis, n_rays, seed, max_seconds):
                        f"one job per source on a {threads}-thread pool, counter RNG mode")
 
 
-def batched_vs_plain(ctx, launches=3):
+def batched_vs_plain(ctx, launches=3, fixed_shape=None):
     """Mixed-size workload: 256 queries over 32 procedural blocks (1k-200k
     tris, 3 or 6 bands, 1-32 sources x 1024-65536 rays, 1-4 listeners,
     1 ms x 1-4 s).  The batched persistent launch (autotuned, one hipGraph
@@ -101,7 +105,11 @@ def batched_vs_plain(ctx, launches=3):
     bc.close()
     b = H.Batch(ctx, Q)
     t0 = time.perf_counter()
-    shape = b.tune(stride=16)
+    if fixed_shape:
+        b.set_shape(*fixed_shape)
+        shape = b.shape
+    else:
+        shape = b.tune(stride=16)
     tune_s = time.perf_counter() - t0
     b.launch()
     b.sync()
@@ -317,7 +325,8 @@ def main():
         torch.cuda.synchronize()
         del x, y
     if not args.no_batch:
-        out["batched_vs_plain"] = batched_vs_plain(ctx)
+        fixed = [int(x) for x in args.batch_shape.split(",")] if args.batch_shape else None
+        out["batched_vs_plain"] = batched_vs_plain(ctx, fixed_shape=fixed)
     if not args.no_serve:
         out["serving"] = serving(ctx)
     if not args.no_cpu_baseline:

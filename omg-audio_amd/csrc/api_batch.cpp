@@ -404,9 +404,26 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
         (void)hipEventDestroy(e0);
         return fail(OMG_ERR_DEVICE, "hipEventCreate");
     }
-    BatchShape best = b->shape;
-    float best_ms = 1e30f;
     hipError_t e = hipSuccess;
+    // best of two sampled launches of one shape
+    auto time_shape = [&](const BatchShape& sh) {
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {
+            e = hipEventRecord(e0, b->stream);
+            if (e == hipSuccess) e = launch_fill_zero_u64(b->ctl.as<unsigned long long>(), 1, b->stream);
+            if (e == hipSuccess) e = launch_trace_batch(B, b->nbm, sh, c->n_cu, false, b->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (e == hipSuccess) ms = std::min(ms, t);
+        }
+        return ms;
+    };
+    // the current shape is kept unless a candidate is 3 % faster, so that
+    // timing noise does not flip between near-equal shapes from run to run
+    BatchShape best = b->shape;
+    float best_ms = time_shape(best);
     // (lanes, stack_lds, regen, width): one lane per ray with regeneration
     // over the binary or the 4-wide tree, one lane or a quad per ray
     // without; each at 2..8 resident waves per SIMD
@@ -418,19 +435,9 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
         if (m[3] == 4 && !b->all4) continue;
         for (uint32_t wps : {2u, 4u, 6u, 8u}) {
             BatchShape sh{m[0], m[1], wps, m[2], m[3]};
-            float ms = 1e30f;
-            for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {
-                e = hipEventRecord(e0, b->stream);
-                if (e == hipSuccess) e = launch_fill_zero_u64(b->ctl.as<unsigned long long>(), 1, b->stream);
-                if (e == hipSuccess) e = launch_trace_batch(B, b->nbm, sh, c->n_cu, false, b->stream);
-                if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-                if (e == hipSuccess) e = hipEventSynchronize(e1);
-                float t = 0.0f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-                if (e == hipSuccess) ms = std::min(ms, t);
-            }
+            float ms = time_shape(sh);
             if (e != hipSuccess) break;
-            if (ms < best_ms) {
+            if (ms < 0.97f * best_ms) {
                 best_ms = ms;
                 best = sh;
             }
