@@ -252,8 +252,15 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         rc = upload(b->jobs, jobs.data(), jobs.size() * sizeof(JobView), b->stream);
     }
     if (!rc) {
+        // queries of one scene are consecutive in the work list, so at any
+        // moment the device walks few BVHs and they stay in L2 / Infinity Cache
+        std::vector<uint32_t> order(n);
+        for (uint32_t i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return b->q[x].scene < b->q[y].scene;
+        });
         b->items_host.reserve(items);
-        for (uint32_t i = 0; i < n; i++)
+        for (uint32_t i : order)
             for (uint32_t s = 0; s < b->q[i].n_src; s++)
                 for (uint32_t r = 0; r < b->q[i].n_rays; r += kBatchItemRays)
                     b->items_host.push_back(
