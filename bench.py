@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 2M-triangle town")
     ap.add_argument("--batch-shape", default="",
                     help="lanes,stack_lds,waves_per_simd,regen,width for the batched launch "
                          "instead of the autotuner's choice (reproducible profiles)")
@@ -197,6 +198,78 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
             "peak_committed_bytes": st["peak_committed_bytes"]}
 
 
+def stress(ctx, dev, launches=2):
+    """Large-scene stress: the ~2M-triangle town (400 buildings with
+    storey slabs, partitions and apertures), 64 sources x 1M rays x 6
+    bands scored at 4 listeners (1 ms x 2 s), 128 bounces, floor 1e-7.  The
+    plain single launch on device buffers and the same job as a one-query
+    batch; bounce-rays from a counted pass (binary walk: the reference's
+    traversal counts)."""
+    import numpy as np
+    import torch
+    import omg_hip as H
+    from omg_hip import scenes
+    t0 = time.perf_counter()
+    mb, src, lis = scenes.c4_workload()
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    scene = H.Scene.from_builder(ctx, mb, nbands=6)
+    build_s = time.perf_counter() - t0
+    info = scene.info()
+    n_rays = 1 << 20
+    params = H.trace_params(n_rays, 6, max_bounces=128, floor_rel=1e-7, seed=0xC4, bin_dt=0.001,
+                            max_time=2.0)
+    counted = H.trace_world(scene, params, src, lis, counters=True)
+    tc = counted.counters
+    n_src, n_lis, nb = src.shape[0], lis.shape[0], H.nbins(2.0, 0.001)
+    d_src = torch.from_numpy(src).to(dev)
+    d_lis = torch.from_numpy(lis).to(dev)
+    d_bins = torch.empty((n_src, n_lis, nb, 6), dtype=torch.float32, device=dev)
+    d_dirs = torch.empty((n_src, n_lis, nb, 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    plain_ms = []
+    for k in range(launches + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        H.trace_world_async(scene, params, d_src.data_ptr(), n_src, d_lis.data_ptr(), n_lis,
+                            d_bins.data_ptr(), d_dirs.data_ptr(), stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if k:
+            plain_ms.append(e0.elapsed_time(e1))
+    got = d_bins.cpu().numpy()
+    rel = float(np.max(np.abs(got - counted.bins)) / max(float(np.max(np.abs(counted.bins))), 1e-30))
+    b = H.Batch(ctx, [H.QuerySpec(scene, params, src, lis)])
+    b.launch()
+    b.sync()
+    batch_s = []
+    for _ in range(launches):
+        t0 = time.perf_counter()
+        b.launch()
+        b.sync()
+        batch_s.append(time.perf_counter() - t0)
+    shape = b.shape
+    b.close()
+    scene.close()
+    br = tc["bounce_rays"]
+    alg = NODE_BYTES * tc["aabb_tests"] + PRIM_BYTES * tc["tri_tests"]
+    pm = float(np.mean(plain_ms)) * 1e-3
+    bs = float(np.mean(batch_s))
+    return {
+        "workload": f"town of {info['n_tris']} tris (400 buildings with interiors), "
+                    f"{n_src} sources x {n_rays} rays x 6 bands at {n_lis} listeners, "
+                    "128 bounces, floor 1e-7",
+        "tris": info["n_tris"], "prims": info["n_prims"], "bvh_nodes": info["n_nodes"],
+        "scene_device_bytes": info["device_bytes"], "generate_s": gen_s, "build_s": build_s,
+        "bounce_rays": br, "aabb_tests": tc["aabb_tests"], "tri_tests": tc["tri_tests"],
+        "plain": {"ms": pm * 1e3, "bounce_rays_per_s": br / pm,
+                  "roofline_frac": alg / pm / 1e9 / HBM_PEAK_GBS},
+        "batched": {"ms": bs * 1e3, "bounce_rays_per_s": br / bs, "shape": shape,
+                    "roofline_frac": alg / bs / 1e9 / HBM_PEAK_GBS},
+        "timed_vs_counted_rel_err": rel,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -329,6 +402,8 @@ def main():
         out["batched_vs_plain"] = batched_vs_plain(ctx, fixed_shape=fixed)
     if not args.no_serve:
         out["serving"] = serving(ctx)
+    if not args.no_stress:
+        out["stress"] = stress(ctx, dev)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -384,3 +384,105 @@ def c5_workload(seed=5, n_scenes=8, tris=(5_000, 500_000), n_walkers=1000, qps=2
                          sources=np.ascontiguousarray(pool[idx]),
                          listener=p.astype(np.float32).reshape(1, 3), seed=seed * 1000003 + len(reqs)))
     return specs, reqs
+
+
+def _slab_cells(mb, mat, x0, x1, y0, y1, z, cell, holes):
+    """A horizontal slab at height z as a grid of quads with (x0, x1, y0, y1) holes removed."""
+    nx = max(1, int(math.ceil((x1 - x0) / cell)))
+    ny = max(1, int(math.ceil((y1 - y0) / cell)))
+    for i in range(nx):
+        ax, bx = x0 + (x1 - x0) * i / nx, x0 + (x1 - x0) * (i + 1) / nx
+        xm = 0.5 * (ax + bx)
+        for j in range(ny):
+            ay, by = y0 + (y1 - y0) * j / ny, y0 + (y1 - y0) * (j + 1) / ny
+            ym = 0.5 * (ay + by)
+            if any(h[0] <= xm <= h[1] and h[2] <= ym <= h[3] for h in holes):
+                continue
+            mb.quad((ax, ay, z), (bx, ay, z), (bx, by, z), (ax, by, z), mat)
+
+
+def town(seed=4, n_buildings=400, target_tris=2_000_000, nbands=6, side=(6.0, 20.0),
+         height=(6.0, 15.0), storey=3.0):
+    """Large-scene stress config: a procedural town of about target_tris
+    triangles.  n_buildings seeded boxes (sides U[side], heights U[height])
+    with interiors: a floor slab per storey with a stair opening, two
+    crossing partition walls per storey with a door each, exterior walls
+    with a door on the ground storey and windows above, a roof; 4 material
+    classes; a ground plane.  Surfaces are gridded so the town totals about
+    target_tris triangles.  Returns (builder, bounds (min, max), boxes)."""
+    rng = SplitMix64(seed)
+    mats = _class_materials(rng, nbands)
+    extent = max(100.0, 14.0 * math.sqrt(n_buildings))
+    boxes = []
+    for _ in range(n_buildings):
+        sx, sy = rng.uniform(*side), rng.uniform(*side)
+        h = rng.uniform(*height)
+        x0 = rng.uniform(0.0, extent - sx)
+        y0 = rng.uniform(0.0, extent - sy)
+        boxes.append(((x0, y0, 0.0), (x0 + sx, y0 + sy, h), rng.randint(0, 3)))
+    area = 0.0
+    for (x0, y0, _), (x1, y1, h), _ in boxes:
+        w, d = x1 - x0, y1 - y0
+        storeys = max(1, int(h / storey))
+        area += 2 * (w + d) * h + w * d            # exterior walls, roof
+        area += (storeys - 1) * w * d              # slabs
+        area += (w + d) * h                        # two partitions
+    cell = max(0.25, math.sqrt(2.0 * area / max(1, target_tris - 2)))
+    mb = MeshBuilder()
+    ids = [mb.material(m) for m in mats]
+    mb.begin_surface()
+    g = extent * 0.1
+    mb.quad((-g, -g, 0.0), (extent + g, -g, 0.0), (extent + g, extent + g, 0.0),
+            (-g, extent + g, 0.0), ids[1])
+    for (x0, y0, _), (x1, y1, h), mi in boxes:
+        m = ids[mi]
+        inner = ids[(mi + 2) % len(ids)]
+        storeys = max(1, int(h / storey))
+        # exterior walls: a door on the ground storey, windows above
+        for axis, plane, lo, hi in [(0, x0, y0, y1), (0, x1, y0, y1), (1, y0, x0, x1),
+                                    (1, y1, x0, x1)]:
+            holes = []
+            if hi - lo > 1.6:
+                u0 = rng.uniform(lo + 0.3, hi - 1.3)
+                holes.append((u0, u0 + 1.0, 0.0, 2.1))
+            for k in range(1, storeys):
+                if hi - lo > 1.8 and rng.uniform() < 0.7:
+                    u0 = rng.uniform(lo + 0.3, hi - 1.5)
+                    holes.append((u0, u0 + 1.2, k * storey + 0.9, k * storey + 2.1))
+            mb.begin_surface()
+            _wall_cells(mb, m, axis, plane, lo, hi, 0.0, h, cell, holes)
+        mb.begin_surface()
+        _slab_cells(mb, m, x0, x1, y0, y1, h, cell, [])
+        # slabs with a stair opening
+        for k in range(1, storeys):
+            z = k * storey
+            if z >= h - 0.5:
+                break
+            sx0 = rng.uniform(x0 + 0.5, max(x0 + 0.6, x1 - 1.7))
+            sy0 = rng.uniform(y0 + 0.5, max(y0 + 0.6, y1 - 3.0))
+            mb.begin_surface()
+            _slab_cells(mb, inner, x0, x1, y0, y1, z, cell, [(sx0, sx0 + 1.2, sy0, sy0 + 2.5)])
+        # two crossing partitions, a door per storey in each
+        xm, ym = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+        for axis, plane, lo, hi in [(1, ym, x0, x1), (0, xm, y0, y1)]:
+            holes = []
+            for k in range(storeys):
+                if hi - lo > 1.6:
+                    u0 = rng.uniform(lo + 0.3, hi - 1.3)
+                    holes.append((u0, u0 + 0.9, k * storey, k * storey + 2.1))
+            mb.begin_surface()
+            _wall_cells(mb, inner, axis, plane, lo, hi, 0.0, h, cell, holes)
+    bounds = (np.array([0.0, 0.0, 0.0], np.float32),
+              np.array([extent, extent, height[1]], np.float32))
+    return mb, bounds, [(b[0], b[1]) for b in boxes]
+
+
+def c4_workload(seed=4):
+    """Large-scene stress: the ~2M-triangle town, 64 sources (half inside
+    buildings, half in the streets) and 4 listeners."""
+    mb, bounds, boxes = town(seed=seed)
+    src = np.vstack([indoor_points(seed * 100 + 1, 32, boxes),
+                     outdoor_points(seed * 100 + 2, 32, bounds, boxes)])
+    lis = np.vstack([indoor_points(seed * 100 + 3, 2, boxes),
+                     outdoor_points(seed * 100 + 4, 2, bounds, boxes)])
+    return mb, src, lis
