@@ -240,6 +240,7 @@ def stress(ctx, dev, launches=2):
     got = d_bins.cpu().numpy()
     rel = float(np.max(np.abs(got - counted.bins)) / max(float(np.max(np.abs(counted.bins))), 1e-30))
     b = H.Batch(ctx, [H.QuerySpec(scene, params, src, lis)])
+    b.tune(stride=64)
     b.launch()
     b.sync()
     batch_s = []
