@@ -416,7 +416,8 @@ typedef struct {
 
 typedef struct {
     uint64_t hbm_budget;        /* bytes: resident scenes + in-flight work arenas */
-    uint32_t slots;             /* batches in flight at once, each on its own stream, 1..8 */
+    uint32_t slots;             /* batches in flight at once, 1..8; with 1-2 slots each slot
+                                   runs its dome fans on a second stream beside the tracer */
     uint32_t max_batch;         /* requests per batch, 1..4096 */
     uint32_t latency_rays;      /* batches of at most this many rays run a DPP quad per ray
                                    over 16-ray items (shorter per-ray chains); 0 = never */

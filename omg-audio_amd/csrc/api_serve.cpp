@@ -290,14 +290,16 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
     B.n_items = (uint32_t)items;
     B.next_item = reinterpret_cast<uint32_t*>(D + o_ctl);
     B.counters = nullptr;
-    if (ng) {
+    // with a side stream the dome fans overlap the tracer; without one they run first
+    hipStream_t ds = S.side ? S.side : st;
+    if (ng && S.side) {
         OMG_HIP_CHECK(hipEventRecord(S.fork, st));
         OMG_HIP_CHECK(hipStreamWaitEvent(S.side, S.fork, 0));
     }
     // a grid no larger than the batch needs leaves the rest of the chip to the other slots
     uint64_t lanes_needed = lat ? items * 64 : rays;
     uint32_t max_blocks = (uint32_t)std::min<uint64_t>((lanes_needed + 255) / 256, 1u << 20);
-    OMG_HIP_CHECK(launch_trace_batch(B, nbm, sh, ctx->n_cu, false, st, max_blocks));
+    if (S.side) OMG_HIP_CHECK(launch_trace_batch(B, nbm, sh, ctx->n_cu, false, st, max_blocks));
     for (uint32_t g = 0; g < ng; g++) {
         DomeArgs A{};
         fill_geom(gscene[g], A.g);
@@ -319,12 +321,13 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         A.dsum = reinterpret_cast<float*>(D + g_d[g]);
         A.status = reinterpret_cast<uint32_t*>(D + o_ctl + 8);
         A.counters = nullptr;
-        OMG_HIP_CHECK(launch_dome(A, false, S.side));
+        OMG_HIP_CHECK(launch_dome(A, false, ds));
     }
-    if (ng) {
+    if (ng && S.side) {
         OMG_HIP_CHECK(hipEventRecord(S.join, S.side));
         OMG_HIP_CHECK(hipStreamWaitEvent(st, S.join, 0));
     }
+    if (!S.side) OMG_HIP_CHECK(launch_trace_batch(B, nbm, sh, ctx->n_cu, false, st, max_blocks));
     OMG_HIP_CHECK(hipMemcpyAsync(S.host_out, D + o_ctl, out_end - o_ctl, hipMemcpyDeviceToHost, st));
     OMG_HIP_CHECK(hipEventRecord(S.ev, st));
     S.out_off = o_ctl;
@@ -392,7 +395,7 @@ void omg_server::loop()
                 finish(si, OMG_OK);
                 did = true;
             } else if (q != hipErrorNotReady) {
-                (void)hipStreamSynchronize(S.side);
+                if (S.side) (void)hipStreamSynchronize(S.side);
                 (void)hipStreamSynchronize(S.stream);
                 finish(si, OMG_ERR_DEVICE);
                 did = true;
@@ -433,7 +436,7 @@ void omg_server::loop()
             if (rc) {
                 S.reqs = batch;
                 S.busy = true;
-                (void)hipStreamSynchronize(S.side);
+                if (S.side) (void)hipStreamSynchronize(S.side);
                 (void)hipStreamSynchronize(S.stream);
                 finish(si, rc);
             }
@@ -497,9 +500,12 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
     s->scenes_bytes = sb;
     s->slots.resize(cfg->slots);
     int rc = OMG_OK;
+    // two streams per slot (tracer, dome) while they fit the device's 4
+    // hardware queues; with more slots each slot runs on one stream
+    const bool side = cfg->slots <= 2;
     for (Slot& S : s->slots) {
         if (hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking) != hipSuccess ||
+            (side && hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking) != hipSuccess) ||
             hipEventCreateWithFlags(&S.ev, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&S.fork, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&S.join, hipEventDisableTiming) != hipSuccess) {
