@@ -56,6 +56,36 @@ def parse():
     return ap.parse_args()
 
 
+def fp32_error_report(mb, src, lis, n_rays, seed, gpu_bins):
+    """The oracle's f64 re-accumulation of the same rays (counter RNG) and the
+    f32 error of the GPU echograms against it, per (source, listener)
+    echogram: max over bins of |gpu - f64| / f64 (bins with energy)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import numpy as np
+    soup = mb.soup()
+    pos, idx = O.dedup_tris(soup["tri_verts"])
+    mats = [(list(m.absorption), m.scattering, list(m.transmission)) for m in soup["materials"]]
+    om = O.Mesh(pos, idx, soup["tri_material"], mats, 3, soup["tri_surface"], soup["tri_max_edge"])
+    g = O.mesh_geom(om)
+    per = []
+    for si in range(src.shape[0]):
+        p = O.trace_params(n_rays, rng_mode=O.RNG_COUNTER, seed=seed, src_index=si, bin_dt=0.001,
+                           max_time=2.0)
+        o = O.trace(g, src[si], lis, p, want_f64=True)
+        for li in range(lis.shape[0]):
+            ref = o["bins64"][li]
+            nz = ref > 0
+            a = gpu_bins[si, li].astype(np.float64)
+            bad = bool(np.any(a[~nz] != 0))
+            per.append(float("inf") if bad else
+                       float(np.max(np.abs(a[nz] - ref[nz]) / ref[nz])) if nz.any() else 0.0)
+    per = np.array(per)
+    return {"echograms": int(per.size), "max_rel": float(per.max()),
+            "median_rel": float(np.median(per)), "tolerance": 1e-4,
+            "per_echogram_max_rel": [float("%.3g" % v) for v in per]}
+
+
 def cpu_baseline(mb, src, lis, n_rays, seed, max_seconds):
     """The oracle (C port of the reference tracer) on the host cores, same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -408,6 +438,7 @@ def main():
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb
+        out["fp32_error_vs_f64"] = fp32_error_report(mb, src, lis, n_rays, seed, got)
     print(json.dumps(out))
 
 

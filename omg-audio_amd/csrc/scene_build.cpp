@@ -288,6 +288,7 @@ int build_scene(const float* positions, uint32_t nv, const uint32_t* indices, ui
     S.max_depth = 0;
     if (prims.empty()) {
         S.empty = true;
+        S.has4 = true;   // nothing to walk at either width
         return 0;
     }
     Builder B;
