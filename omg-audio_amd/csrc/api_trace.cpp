@@ -484,6 +484,12 @@ extern "C" int omg_trace_world(omg_scene* s, const omg_trace_params* p, const fl
     // traversal counters are the reference walk's: counted on the binary tree
     A.width = c->width == 4 && s->has4 && !counters ? 4u : 2u;
     if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
+    if (A.width == 4 && A.lanes == 4 && s->stack4 <= kDeepStack && c->stack_lds == 16) {
+        // the quad walk keeps its whole stack in LDS: 20 KB per block, so
+        // the node prefix shrinks to keep a block within 40 KB
+        A.stack_lds = kDeepStack;
+        A.node_cache = std::min(A.node_cache, 160u);
+    }
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
@@ -513,6 +519,12 @@ extern "C" int omg_trace_world_hits(omg_scene* s, const omg_trace_params* p, con
     A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
     A.width = c->width == 4 && s->has4 ? 4u : 2u;
     if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
+    if (A.width == 4 && A.lanes == 4 && s->stack4 <= kDeepStack && c->stack_lds == 16) {
+        // the quad walk keeps its whole stack in LDS: 20 KB per block, so
+        // the node prefix shrinks to keep a block within 40 KB
+        A.stack_lds = kDeepStack;
+        A.node_cache = std::min(A.node_cache, 160u);
+    }
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
@@ -568,6 +580,12 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
     A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
     A.width = c->width == 4 && s->has4 ? 4u : 2u;
     if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
+    if (A.width == 4 && A.lanes == 4 && s->stack4 <= kDeepStack && c->stack_lds == 16) {
+        // the quad walk keeps its whole stack in LDS: 20 KB per block, so
+        // the node prefix shrinks to keep a block within 40 KB
+        A.stack_lds = kDeepStack;
+        A.node_cache = std::min(A.node_cache, 160u);
+    }
     // the job tables are staged on the context stream, then ordered before the launch
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;

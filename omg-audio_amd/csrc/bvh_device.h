@@ -685,6 +685,96 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
     return H.result(t_out, tri_out, flags);
 }
 
+// The quad walk with a stack deep enough for the whole tree (SK at least the
+// tree's worst-case stack, checked on the host), so nothing spills to
+// scratch and the four lanes push in parallel: lane q ranks its own child
+// among the quad's hits (by entry distance, then slot - the order the
+// sorting walk produces), writes it at stack depth sp + nhit - 1 - rank
+// unless it is the nearest, and the nearest child's reference and distance
+// reach every lane through an OR over the quad (only the nearest lane
+// contributes non-zero bits; node 0 is the root, never a child).
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const Quad4Trav& L, f3 o,
+                                                        f3 d, float& t_out, uint32_t& tri_out,
+                                                        uint32_t& flags, TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int sp = 0;
+    uint32_t cur = G.root4;
+    float cur_t = 0.0f;
+    BestHit H;
+    const uint32_t q = L.q;
+    const uint32_t qshift = (__lane_id() & 63u) & ~3u;
+    while (true) {
+        if (!(H.found() && cur_t >= H.best)) {
+            if (cur & kLeafBit) {
+                uint32_t start = cur & kLeafStartMask;
+                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+                uint32_t hit_q = 0, tri_q = 0;
+                float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+                if (q < cnt_p) {
+                    Prim p = load_prim(G.prims, start + q);
+                    tri_q = p.tri;
+                    hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+                }
+                if (COUNT && q == 0) cnt.tri += cnt_p;
+                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+            } else {
+                float4 lo, hi;
+                if (cur < L.nc) {
+                    const lds_float4* src = L.nodes + 8 * cur + 2 * q;
+                    lo = lds_ld4(src);
+                    hi = lds_ld4(src + 1);
+                } else {
+                    const float4* src = reinterpret_cast<const float4*>(G.nodes4 + cur) + 2 * q;
+                    lo = src[0];
+                    hi = src[1];
+                }
+                uint32_t ref = __float_as_uint(lo.w);
+                const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
+                float te = 0.0f;
+                bool h = ref != kNoChild &&
+                         ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te);
+                if (COUNT && q == 0) {
+                    uint32_t valid = (uint32_t)(__ballot(ref != kNoChild) >> qshift) & 0xFu;
+                    cnt.aabb += __popc(valid);
+                }
+                const uint32_t hm = (uint32_t)(__ballot(h) >> qshift) & 0xFu;
+                const int nhit = __popc(hm);
+                if (nhit > 0) {
+                    // rank of this lane's child among the quad's hits
+                    const float t0 = qbf<0>(te), t1 = qbf<1>(te), t2 = qbf<2>(te), t3 = qbf<3>(te);
+                    int rank = 0;
+                    rank += ((hm & 1u) && (t0 < te || (t0 == te && 0u < q))) ? 1 : 0;
+                    rank += ((hm & 2u) && (t1 < te || (t1 == te && 1u < q))) ? 1 : 0;
+                    rank += ((hm & 4u) && (t2 < te || (t2 == te && 2u < q))) ? 1 : 0;
+                    rank += ((hm & 8u) && (t3 < te || (t3 == te && 3u < q))) ? 1 : 0;
+                    if (h && rank > 0) {
+                        const int pos = sp + nhit - 1 - rank;
+                        L.st_ref[pos * L.nq + L.quad] = ref;
+                        L.st_t[pos * L.nq + L.quad] = te;
+                    }
+                    const bool near = h && rank == 0;
+                    const uint32_t nr = near ? ref : 0u, nt = near ? __float_as_uint(te) : 0u;
+                    cur = qb<0>(nr) | qb<1>(nr) | qb<2>(nr) | qb<3>(nr);
+                    cur_t = __uint_as_float(qb<0>(nt) | qb<1>(nt) | qb<2>(nt) | qb<3>(nt));
+                    sp += nhit - 1;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        cur = L.st_ref[sp * L.nq + L.quad];
+        cur_t = L.st_t[sp * L.nq + L.quad];
+    }
+    return H.result(t_out, tri_out, flags);
+}
+
 // One crossing of a segment: position along it (0..1), original triangle,
 // authored surface and material.
 struct SegHit {

@@ -51,6 +51,11 @@ struct TraceArgs : JobView {
     float* hit_t;            // [rays][max_bounces] hit distance per bounce, or null
 };
 
+// LDS stack entries per ray of the quad 4-wide walk that never spills: used
+// when the tree's worst-case stack fits (every scene measured so far needs
+// at most 39)
+constexpr uint32_t kDeepStack = 40;
+
 // One unit of batched work: up to kBatchItemRays consecutive rays of one
 // source of one query.
 constexpr uint32_t kBatchItemRays = 64;

@@ -110,7 +110,10 @@ __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Quad4T
                                                  float& t, uint32_t& tri, uint32_t& fl,
                                                  TravCount& c)
 {
-    return mesh_raycast4_quad<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+    if constexpr (SK >= kDeepStack)
+        return mesh_raycast4_quad_deep<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+    else
+        return mesh_raycast4_quad<COUNT, SK>(G, L, o, d, t, tri, fl, c);
 }
 
 template <bool MESH, bool COUNT, int SK, class Trav>
@@ -575,11 +578,15 @@ static hipError_t launch_trace_t(const TraceArgs& A, hipStream_t s)
     return launch_trace_l<NBM, MESH, COUNT, SK, 1, 2>(A, s);
 }
 
-// 4-wide walks keep 16 stack entries per ray in LDS
+// 4-wide walks keep 16 stack entries per ray in LDS, or, on a quad, the
+// whole worst-case stack (kDeepStack entries)
 template <int NBM, bool COUNT>
 static hipError_t launch_trace_w4(const TraceArgs& A, hipStream_t s)
 {
-    if (A.lanes == 4) return launch_trace_l<NBM, true, COUNT, 16, 4, 4>(A, s);
+    if (A.lanes == 4) {
+        if (A.stack_lds == kDeepStack) return launch_trace_l<NBM, true, COUNT, kDeepStack, 4, 4>(A, s);
+        return launch_trace_l<NBM, true, COUNT, 16, 4, 4>(A, s);
+    }
     return launch_trace_l<NBM, true, COUNT, 16, 1, 4>(A, s);
 }
 
