@@ -196,7 +196,7 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
     R = [dict(scene=pool[r["scene"]], params=H.trace_params(r["n_rays"], 3, seed=r["seed"]),
               sources=r["sources"], listeners=r["listener"], dome=True) for r in reqs]
     arr0 = np.array([r["t"] for r in reqs])
-    shape = dict(lanes=1, stack_lds=8, waves_per_simd=8, regen=1, width=4)
+    shape = dict(lanes=1, stack_lds=12, waves_per_simd=6, regen=1, width=4)
     srv = H.Server(ctx, pool, hbm_budget=64 << 30, slots=2, max_batch=64, shape=shape,
                    latency_rays=1 << 20)
     srv.replay(R[:100], arr0[:100])

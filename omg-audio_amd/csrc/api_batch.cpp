@@ -48,6 +48,7 @@ struct omg_batch {
     uint64_t class_key = 0;
     BatchShape shape{1, 16, 4, 1, 2};
     bool all4 = true;              // every scene has its 4-wide tree
+    bool deep = true;              // ... and its stack fits kDeepStack entries
     float shape_ms = 0.0f;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;
@@ -59,7 +60,7 @@ namespace {
 
 BatchArgs batch_args(omg_batch* b, const BatchItem* items, uint32_t n_items)
 {
-    BatchArgs B;
+    BatchArgs B{};
     B.jobs = b->jobs.as<JobView>();
     B.items = items;
     B.n_items = n_items;
@@ -89,7 +90,10 @@ int capture(omg_batch* b)
     if (e == hipSuccess) e = launch_fill_zero(b->dirs.as<float>(), b->n_dirs_f, b->stream);
     // traversal counters are the reference walk's: a counting batch walks the binary tree
     BatchShape sh = b->shape;
-    if (b->flags & OMG_BATCH_COUNTERS) sh.width = 2;
+    if (b->flags & OMG_BATCH_COUNTERS) {
+        sh.width = 2;
+        if (sh.stack_lds == kDeepStack) sh.stack_lds = 16;
+    }
     if (e == hipSuccess)
         e = launch_trace_batch(B, b->nbm, sh, c->n_cu, (b->flags & OMG_BATCH_COUNTERS) != 0,
                                b->stream);
@@ -122,15 +126,6 @@ void batch_free(omg_batch* b)
 
 }  // namespace
 
-bool omg::valid_batch_shape(const BatchShape& s)
-{
-    if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 1) return false;
-    if (s.width != 2 && s.width != 4) return false;
-    if (s.regen) return s.lanes == 1 && (s.stack_lds == 8 || s.stack_lds == 12 || s.stack_lds == 16);
-    if (s.width == 4) return (s.lanes == 1 || s.lanes == 4) && s.stack_lds == 16;
-    return (s.lanes == 1 || s.lanes == 4) && (s.stack_lds == 16 || s.stack_lds == 32);
-}
-
 extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n, uint32_t flags,
                                 omg_batch** out)
 {
@@ -142,7 +137,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
     uint32_t nbm = 1;
     uint64_t rays = 0, items = 0;
     double lg_prims = 0.0;
-    bool all4 = true;
+    bool all4 = true, deep = true;
     for (uint32_t i = 0; i < n; i++) {
         const omg_query& Q = queries[i];
         if (!Q.scene || Q.scene->ctx != c || !Q.sources || !Q.listeners || (Q.n_ovl && !Q.overlays))
@@ -151,6 +146,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         if (rc) return fail(rc, "query " + std::to_string(i) + ": " + omg_last_error());
         nbm = std::max(nbm, Q.scene->nbands);
         all4 = all4 && Q.scene->has4;
+        deep = deep && Q.scene->has4 && Q.scene->stack4 <= kDeepStack;
         uint64_t r = (uint64_t)Q.params.n_rays * Q.n_src;
         rays += r;
         items += (uint64_t)Q.n_src * ((Q.params.n_rays + kBatchItemRays - 1) / kBatchItemRays);
@@ -165,6 +161,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
     b->nbm = nbm;
     b->rays = rays;
     b->all4 = all4;
+    b->deep = deep;
     b->q.resize(n);
     // host staging of every query's tables: mats | overlays | sources | listeners
     std::vector<size_t> off_mats(n), off_ovl(n), off_src(n), off_lis(n);
@@ -275,9 +272,11 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         b->class_key = ((uint64_t)(nbm <= 3 ? 3 : nbm <= 6 ? 6 : 8) << 16) |
                        ((uint64_t)log2_ceil(rays) << 8) | lg;
         // untuned default: the regenerating walk (4-wide where every scene has it)
-        b->shape = all4 ? BatchShape{1, 8, 8, 1, 4} : BatchShape{1, 8, 8, 1, 2};
+        b->shape = all4 ? BatchShape{1, 12, 6, 1, 4} : BatchShape{1, 8, 8, 1, 2};
         auto it = c->tuned.find(b->class_key);
-        if (it != c->tuned.end() && (it->second.width == 2 || all4)) b->shape = it->second;
+        if (it != c->tuned.end() && (it->second.width == 2 || all4) &&
+            (it->second.stack_lds != kDeepStack || deep))
+            b->shape = it->second;
         rc = capture(b);
     }
     if (rc) {
@@ -379,9 +378,12 @@ extern "C" int omg_batch_set_shape(omg_batch* b, const omg_batch_shape* s)
     BatchShape sh{s->lanes, s->stack_lds, s->waves_per_simd, s->regen, s->width};
     if (!valid_batch_shape(sh))
         return fail(OMG_ERR_INVALID,
-                    "shape: lanes 1|4 with stack_lds 16|32 (width 2) or 16 (width 4), or regen 1 "
-                    "with lanes 1, stack_lds 8|12|16 and width 2|4; waves_per_simd 1..8");
+                    "shape: lanes 1|4 with stack_lds 16|32 (width 2) or 16 (width 4); regen 1 "
+                    "with lanes 1, stack_lds 8|12|16 and width 2|4, or lanes 4, stack_lds 16 and "
+                    "width 2; lanes 4 with stack_lds 40 and width 4; waves_per_simd 1..8");
     if (sh.width == 4 && !b->all4) return fail(OMG_ERR_INVALID, "a scene of the batch has no 4-wide tree");
+    if (sh.stack_lds == kDeepStack && !b->deep)
+        return fail(OMG_ERR_INVALID, "a scene of the batch needs more than 40 stack entries");
     OMG_HIP_CHECK(hipSetDevice(b->ctx->device));
     int rc = omg_batch_sync(b);
     if (rc) return rc;
@@ -431,15 +433,18 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
     // timing noise does not flip between near-equal shapes from run to run
     BatchShape best = b->shape;
     float best_ms = time_shape(best);
-    // (lanes, stack_lds, regen, width): one lane per ray with regeneration
-    // over the binary or the 4-wide tree, one lane or a quad per ray
-    // without; each at 2..8 resident waves per SIMD
-    const uint32_t modes[12][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
+    // (lanes, stack_lds, regen, width): a lane or a quad per ray, with and
+    // without regeneration, over the binary or the 4-wide tree; each at
+    // 2..8 resident waves per SIMD
+    const uint32_t modes[15][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
                                    {1u, 8u, 1u, 4u},  {1u, 12u, 1u, 4u}, {1u, 16u, 1u, 4u},
-                                   {1u, 16u, 0u, 2u}, {1u, 32u, 0u, 2u}, {4u, 16u, 0u, 2u},
-                                   {4u, 32u, 0u, 2u}, {1u, 16u, 0u, 4u}, {4u, 16u, 0u, 4u}};
+                                   {4u, 16u, 1u, 2u}, {4u, 40u, 1u, 4u}, {1u, 16u, 0u, 2u},
+                                   {1u, 32u, 0u, 2u}, {4u, 16u, 0u, 2u}, {4u, 32u, 0u, 2u},
+                                   {1u, 16u, 0u, 4u}, {4u, 16u, 0u, 4u}, {4u, 40u, 0u, 4u}};
+    static_assert(kDeepStack == 40, "tuner modes name the deep stack");
     for (const auto& m : modes) {
         if (m[3] == 4 && !b->all4) continue;
+        if (m[1] == kDeepStack && !b->deep) continue;
         for (uint32_t wps : {2u, 4u, 6u, 8u}) {
             BatchShape sh{m[0], m[1], wps, m[2], m[3]};
             float ms = time_shape(sh);

@@ -92,6 +92,7 @@ struct omg_server {
     omg_ctx* ctx = nullptr;
     omg_server_config cfg{};
     BatchShape shape{};
+    bool deep = false;  // every scene's 4-wide stack fits kDeepStack entries
     std::vector<omg_scene*> scenes;
     uint64_t scenes_bytes = 0;
     std::vector<Slot> slots;
@@ -167,11 +168,14 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         for (uint32_t i : gmem[g]) g_eyes[g] += batch[i]->n_lis;
 
     // small batches run in latency mode: a DPP quad per ray over 16-ray items
+    // (the whole traversal stack in LDS where the trees allow it)
     uint64_t rays = 0;
     for (uint32_t i = 0; i < n; i++) rays += (uint64_t)batch[i]->n_src * batch[i]->p.n_rays;
     const bool lat = cfg.latency_rays && rays <= cfg.latency_rays;
     const uint32_t item_rays = lat ? kLatencyItemRays : kBatchItemRays;
-    const BatchShape sh = lat ? BatchShape{4, 16, 8, 0, shape.width} : shape;
+    const BatchShape sh = !lat                        ? shape
+                          : deep && shape.width == 4 ? BatchShape{4, kDeepStack, 8, 0, 4}
+                                                     : BatchShape{4, 16, 8, 0, shape.width};
 
     // inputs
     std::vector<size_t> o_mats(n), o_src(n), o_lis(n), g_eye_off(ng);
@@ -284,7 +288,7 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
     hipStream_t st = S.stream;
     OMG_HIP_CHECK(hipMemcpyAsync(D, H, o_ctl, hipMemcpyHostToDevice, st));
     OMG_HIP_CHECK(launch_fill_zero(reinterpret_cast<float*>(D + o_ctl), (out_end - o_ctl) / 4, st));
-    BatchArgs B;
+    BatchArgs B{};
     B.jobs = reinterpret_cast<const JobView*>(D + o_jobs);
     B.items = reinterpret_cast<const BatchItem*>(D + o_items);
     B.n_items = (uint32_t)items;
@@ -297,7 +301,7 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         OMG_HIP_CHECK(hipStreamWaitEvent(S.side, S.fork, 0));
     }
     // a grid no larger than the batch needs leaves the rest of the chip to the other slots
-    uint64_t lanes_needed = lat ? items * 64 : rays;
+    uint64_t lanes_needed = lat ? items * 64 : rays * sh.lanes;
     uint32_t max_blocks = (uint32_t)std::min<uint64_t>((lanes_needed + 255) / 256, 1u << 20);
     if (S.side) OMG_HIP_CHECK(launch_trace_batch(B, nbm, sh, ctx->n_cu, false, st, max_blocks));
     for (uint32_t g = 0; g < ng; g++) {
@@ -479,9 +483,14 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
     BatchShape sh{cfg->shape.lanes, cfg->shape.stack_lds, cfg->shape.waves_per_simd, cfg->shape.regen,
                   cfg->shape.width};
     if (!valid_batch_shape(sh)) return fail(OMG_ERR_INVALID, "launch shape");
-    for (uint32_t i = 0; i < n_scenes; i++)
+    bool deep = n_scenes > 0;
+    for (uint32_t i = 0; i < n_scenes; i++) {
         if (sh.width == 4 && scenes[i] && !scenes[i]->has4)
             return fail(OMG_ERR_INVALID, "a scene has no 4-wide tree");
+        deep = deep && scenes[i] && scenes[i]->has4 && scenes[i]->stack4 <= kDeepStack;
+    }
+    if (sh.stack_lds == kDeepStack && !deep)
+        return fail(OMG_ERR_INVALID, "a scene needs more than 40 stack entries");
     const omg_dome_params& dp = cfg->dome;
     if (dp.n_fan == 0 || dp.n_rays == 0 || dp.n_rays > dp.n_fan || dp.max_events == 0 || dp.max_events > 64)
         return fail(OMG_ERR_INVALID, "dome parameters");
@@ -496,6 +505,7 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
     s->ctx = c;
     s->cfg = *cfg;
     s->shape = sh;
+    s->deep = deep;
     s->scenes.assign(scenes, scenes + n_scenes);
     s->scenes_bytes = sb;
     s->slots.resize(cfg->slots);

@@ -10,30 +10,61 @@
 
 namespace omg {
 
-// Mesh::ray_aabb (mesh.rs:409-426) on one box; t_enter is the entry distance.
+// Global memory pointers.  A pointer that a kernel reads out of memory (the
+// geometry and tables of a batch's job views) is generic to the compiler,
+// which then emits FLAT accesses: they wait on both the vector-memory and
+// the LDS counters and cannot use the global addressing modes.  Every
+// buffer behind these pointers is device memory, so the casts are exact.
+// (The qualifier exists in the device pass only: the host pass parses these
+// functions without a global address space to convert to.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define OMG_GLOBAL __attribute__((address_space(1)))
+#else
+#define OMG_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ const OMG_GLOBAL T* glb(const T* p)
+{
+    return (const OMG_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ OMG_GLOBAL T* glb(T* p)
+{
+    return (OMG_GLOBAL T*)p;
+}
+// Hardware f32 atomic add on device memory (no return value used).
+__device__ __forceinline__ void atomic_add_f32(float* p, float v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    (void)__builtin_amdgcn_global_atomic_fadd_f32(glb(p), v);
+#endif
+}
+
+// One slab of the box test: narrows [t0, t1] to the box's extent along one axis.
+__device__ __forceinline__ void ray_slab(float lo, float hi, float o, float inv, float& t0,
+                                         float& t1)
+{
+    float ta = (lo - o) * inv, tb = (hi - o) * inv;
+    if (ta > tb) { float s = ta; ta = tb; tb = s; }
+    t0 = fmaxf(t0, ta);
+    t1 = fminf(t1, tb);
+}
+
+// Mesh::ray_aabb (mesh.rs:409-426) on one box; t_enter is the entry
+// distance.  The reference returns at the first axis where t0 > t1; here
+// the three slabs are always narrowed and t0 > t1 is tested once.  Both
+// give the same answer and the same t_enter: t0 starts at 0 and t1 at tmax
+// (never NaN), fmaxf/fminf drop a NaN operand, so t0 never decreases and
+// t1 never increases, and once t0 > t1 holds it holds to the end.
 __device__ __forceinline__ bool ray_aabb(const float bmin[3], const float bmax[3], f3 o, f3 inv,
                                          float tmax, float& t_enter)
 {
     float t0 = 0.0f, t1 = tmax;
-    float ta = (bmin[0] - o.x) * inv.x, tb = (bmax[0] - o.x) * inv.x;
-    if (ta > tb) { float s = ta; ta = tb; tb = s; }
-    t0 = fmaxf(t0, ta);
-    t1 = fminf(t1, tb);
-    if (t0 > t1) return false;
-    ta = (bmin[1] - o.y) * inv.y;
-    tb = (bmax[1] - o.y) * inv.y;
-    if (ta > tb) { float s = ta; ta = tb; tb = s; }
-    t0 = fmaxf(t0, ta);
-    t1 = fminf(t1, tb);
-    if (t0 > t1) return false;
-    ta = (bmin[2] - o.z) * inv.z;
-    tb = (bmax[2] - o.z) * inv.z;
-    if (ta > tb) { float s = ta; ta = tb; tb = s; }
-    t0 = fmaxf(t0, ta);
-    t1 = fminf(t1, tb);
-    if (t0 > t1) return false;
+    ray_slab(bmin[0], bmax[0], o.x, inv.x, t0, t1);
+    ray_slab(bmin[1], bmax[1], o.y, inv.y, t0, t1);
+    ray_slab(bmin[2], bmax[2], o.z, inv.z, t0, t1);
     t_enter = t0;
-    return true;
+    return !(t0 > t1);
 }
 
 // Mesh::ray_tri_packed (mesh.rs:437-456): two-sided Moller-Trumbore.
@@ -62,7 +93,7 @@ __device__ __forceinline__ bool ray_tri(const Prim& p, f3 o, f3 d, float& t_out,
 
 __device__ __forceinline__ Prim load_prim(const Prim* prims, uint32_t k)
 {
-    const float4* q = reinterpret_cast<const float4*>(prims + k);
+    const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(prims + k));
     float4 a = q[0], b = q[1], c = q[2];
     Prim p;
     p.a[0] = a.x; p.a[1] = a.y; p.a[2] = a.z; p.tri = __float_as_uint(a.w);
@@ -141,7 +172,7 @@ __device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, floa
                 if (ray_tri(p, o, d, t, u, v)) H.offer(t, p.tri, u, v);
             }
         } else {
-            const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
+            const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + ref);
             float4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
             const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
             const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
@@ -212,7 +243,7 @@ __device__ __forceinline__ void load_node(const GeomArgs& G, const LdsTrav& L, u
         n2 = lds_ld4(q + 2);
         n3 = lds_ld4(q + 3);
     } else {
-        const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
+        const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + ref);
         n0 = q[0]; n1 = q[1]; n2 = q[2]; n3 = q[3];
     }
 }
@@ -371,7 +402,7 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
                     lo = lds_ld4(src + 2 * half);
                     hi = lds_ld4(src + 2 * half + 1);
                 } else {
-                    const float4* src = reinterpret_cast<const float4*>(G.nodes + cur);
+                    const OMG_GLOBAL float4* src = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + cur);
                     lo = src[2 * half];
                     hi = src[2 * half + 1];
                 }
@@ -493,12 +524,11 @@ __device__ __forceinline__ int order_children(const float4 lo[4], const float4 h
     for (int j = 0; j < 4; j++) {
         uint32_t ref = __float_as_uint(lo[j].w);
         const float bmin[3] = {lo[j].x, lo[j].y, lo[j].z}, bmax[3] = {hi[j].x, hi[j].y, hi[j].z};
+        // an empty slot holds a zero box: its test runs but never counts
         float te = 0.0f;
-        bool h = false;
-        if (ref != kNoChild) {
-            tested++;
-            h = ray_aabb(bmin, bmax, o, inv, tmax, te);
-        }
+        const bool valid = ref != kNoChild;
+        bool h = ray_aabb(bmin, bmax, o, inv, tmax, te) && valid;
+        tested += valid ? 1u : 0u;
         c[j].t = h ? te : kMiss;
         c[j].ref = ref;
         c[j].slot = (uint32_t)j;
@@ -508,83 +538,111 @@ __device__ __forceinline__ int order_children(const float4 lo[4], const float4 h
     return nhit;
 }
 
+// Resumable state of the 4-wide walk of one lane: the node in hand, its
+// entry distance, the stack depth and the nearest hit so far.
+struct Walk4 {
+    f3 inv;
+    uint32_t cur;
+    float cur_t;
+    int sp;
+    BestHit H;
+};
+
+// Starts a walk; false when there is nothing to walk (an empty mesh).
+__device__ __forceinline__ bool walk4_begin(const GeomArgs& G, Walk4& W, f3 d)
+{
+    W.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    W.cur = G.root4;
+    W.cur_t = 0.0f;
+    W.sp = 0;
+    W.H = BestHit();
+    return !G.empty;
+}
+
+// One step of the nearest-hit walk over the 4-wide nodes (one node tested,
+// or one leaf's primitives, or one pop); false once the walk has ended.
+// The stack's top SK entries live in LDS, deeper ones in ovf.
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L, Walk4& W, f3 o,
+                                           f3 d, uint32_t (&ovf_ref)[kStackCap - SK],
+                                           float (&ovf_t)[kStackCap - SK], TravCount& cnt)
+{
+    if (!(W.H.found() && W.cur_t >= W.H.best)) {
+        if (W.cur & kLeafBit) {
+            uint32_t start = W.cur & kLeafStartMask;
+            uint32_t cnt_p = (W.cur >> kLeafCountShift) & 7u;
+            for (uint32_t k = start; k < start + cnt_p; k++) {
+                Prim p = load_prim(G.prims, k);
+                if (COUNT) cnt.tri++;
+                float t, u, v;
+                if (ray_tri(p, o, d, t, u, v)) W.H.offer(t, p.tri, u, v);
+            }
+        } else {
+            float4 lo[4], hi[4];
+            if (W.cur < L.nc) {
+                const lds_float4* q = L.nodes + 8 * W.cur;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    lo[j] = lds_ld4(q + 2 * j);
+                    hi[j] = lds_ld4(q + 2 * j + 1);
+                }
+            } else {
+                const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes4) + W.cur);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    lo[j] = q[2 * j];
+                    hi[j] = q[2 * j + 1];
+                }
+            }
+            Cand4 c[4];
+            uint32_t tested;
+            int nhit = order_children(lo, hi, o, W.inv, W.H.found() ? W.H.best : FLT_MAX, c, tested);
+            if (COUNT) cnt.aabb += tested;
+            if (nhit > 0) {
+                auto push = [&](uint32_t ref, float t) {
+                    if (W.sp < SK) {
+                        L.st_ref[W.sp * L.bs + L.lane] = ref;
+                        L.st_t[W.sp * L.bs + L.lane] = t;
+                    } else {
+                        ovf_ref[W.sp - SK] = ref;
+                        ovf_t[W.sp - SK] = t;
+                    }
+                    W.sp++;
+                };
+                if (nhit > 3) push(c[3].ref, c[3].t);
+                if (nhit > 2) push(c[2].ref, c[2].t);
+                if (nhit > 1) push(c[1].ref, c[1].t);
+                W.cur = c[0].ref;
+                W.cur_t = c[0].t;
+                return true;
+            }
+        }
+    }
+    if (W.sp == 0) return false;
+    W.sp--;
+    if (W.sp < SK) {
+        W.cur = L.st_ref[W.sp * L.bs + L.lane];
+        W.cur_t = L.st_t[W.sp * L.bs + L.lane];
+    } else {
+        W.cur = ovf_ref[W.sp - SK];
+        W.cur_t = ovf_t[W.sp - SK];
+    }
+    return true;
+}
+
 // Nearest hit over the 4-wide nodes, one lane per ray, stack top in LDS.
 template <bool COUNT, int SK>
 __device__ __forceinline__ bool mesh_raycast4_lds(const GeomArgs& G, const Lds4Trav& L, f3 o, f3 d,
                                                   float& t_out, uint32_t& tri_out, uint32_t& flags,
                                                   TravCount& cnt)
 {
-    if (G.empty) return false;
-    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     uint32_t ovf_ref[kStackCap - SK];
     float ovf_t[kStackCap - SK];
-    int sp = 0;
-    uint32_t cur = G.root4;
-    float cur_t = 0.0f;
-    BestHit H;
-    auto push = [&](uint32_t ref, float t) {
-        if (sp < SK) {
-            L.st_ref[sp * L.bs + L.lane] = ref;
-            L.st_t[sp * L.bs + L.lane] = t;
-        } else {
-            ovf_ref[sp - SK] = ref;
-            ovf_t[sp - SK] = t;
-        }
-        sp++;
-    };
-    while (true) {
-        if (!(H.found() && cur_t >= H.best)) {
-            if (cur & kLeafBit) {
-                uint32_t start = cur & kLeafStartMask;
-                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
-                for (uint32_t k = start; k < start + cnt_p; k++) {
-                    Prim p = load_prim(G.prims, k);
-                    if (COUNT) cnt.tri++;
-                    float t, u, v;
-                    if (ray_tri(p, o, d, t, u, v)) H.offer(t, p.tri, u, v);
-                }
-            } else {
-                float4 lo[4], hi[4];
-                if (cur < L.nc) {
-                    const lds_float4* q = L.nodes + 8 * cur;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        lo[j] = lds_ld4(q + 2 * j);
-                        hi[j] = lds_ld4(q + 2 * j + 1);
-                    }
-                } else {
-                    const float4* q = reinterpret_cast<const float4*>(G.nodes4 + cur);
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        lo[j] = q[2 * j];
-                        hi[j] = q[2 * j + 1];
-                    }
-                }
-                Cand4 c[4];
-                uint32_t tested;
-                int nhit = order_children(lo, hi, o, inv, H.found() ? H.best : FLT_MAX, c, tested);
-                if (COUNT) cnt.aabb += tested;
-                if (nhit > 0) {
-                    if (nhit > 3) push(c[3].ref, c[3].t);
-                    if (nhit > 2) push(c[2].ref, c[2].t);
-                    if (nhit > 1) push(c[1].ref, c[1].t);
-                    cur = c[0].ref;
-                    cur_t = c[0].t;
-                    continue;
-                }
-            }
-        }
-        if (sp == 0) break;
-        sp--;
-        if (sp < SK) {
-            cur = L.st_ref[sp * L.bs + L.lane];
-            cur_t = L.st_t[sp * L.bs + L.lane];
-        } else {
-            cur = ovf_ref[sp - SK];
-            cur_t = ovf_t[sp - SK];
-        }
+    Walk4 W;
+    if (!walk4_begin(G, W, d)) return false;
+    while (walk4_step<COUNT, SK>(G, L, W, o, d, ovf_ref, ovf_t, cnt)) {
     }
-    return H.result(t_out, tri_out, flags);
+    return W.H.result(t_out, tri_out, flags);
 }
 
 // The same walk on a DPP quad: lane q loads and tests child q (32 B) and
@@ -641,15 +699,15 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
                     lo = lds_ld4(src);
                     hi = lds_ld4(src + 1);
                 } else {
-                    const float4* src = reinterpret_cast<const float4*>(G.nodes4 + cur) + 2 * q;
+                    const OMG_GLOBAL float4* src = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes4) + cur) + 2 * q;
                     lo = src[0];
                     hi = src[1];
                 }
                 uint32_t ref = __float_as_uint(lo.w);
                 const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
                 float te = 0.0f;
-                bool h = ref != kNoChild &&
-                         ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te);
+                bool h = ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te) &&
+                         ref != kNoChild;
                 float tk = h ? te : kMiss;
                 Cand4 c[4];
                 c[0] = Cand4{qbf<0>(tk), qb<0>(ref), 0u};
@@ -730,15 +788,15 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                     lo = lds_ld4(src);
                     hi = lds_ld4(src + 1);
                 } else {
-                    const float4* src = reinterpret_cast<const float4*>(G.nodes4 + cur) + 2 * q;
+                    const OMG_GLOBAL float4* src = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes4) + cur) + 2 * q;
                     lo = src[0];
                     hi = src[1];
                 }
                 uint32_t ref = __float_as_uint(lo.w);
                 const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
                 float te = 0.0f;
-                bool h = ref != kNoChild &&
-                         ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te);
+                bool h = ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te) &&
+                         ref != kNoChild;
                 if (COUNT && q == 0) {
                     uint32_t valid = (uint32_t)(__ballot(ref != kNoChild) >> qshift) & 0xFu;
                     cnt.aabb += __popc(valid);
@@ -836,7 +894,7 @@ __device__ __forceinline__ int segment_hits(const GeomArgs& G, f3 a, f3 b, SegHi
                 }
             }
         } else {
-            const float4* q = reinterpret_cast<const float4*>(G.nodes + ref);
+            const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + ref);
             float4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
             const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
             const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};

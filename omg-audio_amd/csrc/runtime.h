@@ -121,5 +121,4 @@ void job_scalars(const omg_trace_params* p, uint32_t nbands, uint32_t n_ovl, Job
 // drops one scene/batch/server reference; tears the context down after omg_shutdown
 void ctx_release(omg_ctx* c);
 void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins);
-bool valid_batch_shape(const BatchShape& s);
 }  // namespace omg

@@ -78,12 +78,28 @@ struct BatchArgs {
 // Launch shape of the batched kernel (chosen by the autotuner).
 struct BatchShape {
     uint32_t lanes;           // lanes per ray: 1 or 4
-    uint32_t stack_lds;       // stack entries per ray in LDS: 16 or 32; with regen 8, 12 or 16
+    uint32_t stack_lds;       // stack entries per ray in LDS (see valid_batch_shape)
     uint32_t waves_per_simd;  // resident waves per SIMD the grid is sized for
-    uint32_t regen;           // 1: dead lanes take new rays between bounces (lanes = 1 only)
-    uint32_t width;           // BVH width walked: 2 or 4 (every scene of the batch has nodes4;
-                              // without regen, width 4 keeps 16 stack entries)
+    uint32_t regen;           // 1: finished rays are replaced between bounces
+    uint32_t width;           // BVH width walked: 2 or 4 (every scene of the batch has nodes4)
 };
+
+// The compiled shapes.  Without regen: lanes 1|4 with 16|32 stack entries
+// over the binary tree, 16 over the 4-wide tree, or a quad with kDeepStack
+// entries over the 4-wide tree.  With regen: one lane per ray with 8|12|16
+// entries over either tree, or a quad per ray with 16 (binary tree) or
+// kDeepStack (4-wide tree).  A kDeepStack shape is valid only for scenes
+// whose 4-wide tree needs at most kDeepStack stack entries.
+inline bool valid_batch_shape(const BatchShape& s)
+{
+    if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 1) return false;
+    if (s.width != 2 && s.width != 4) return false;
+    if (s.lanes == 4 && s.width == 4 && s.stack_lds == kDeepStack) return true;
+    if (s.regen && s.lanes == 4) return s.width == 2 && s.stack_lds == 16;
+    if (s.regen) return s.lanes == 1 && (s.stack_lds == 8 || s.stack_lds == 12 || s.stack_lds == 16);
+    if (s.width == 4) return (s.lanes == 1 || s.lanes == 4) && s.stack_lds == 16;
+    return (s.lanes == 1 || s.lanes == 4) && (s.stack_lds == 16 || s.stack_lds == 32);
+}
 
 hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s);
 // persistent work-stealing launch over n_cu compute units; nbm = the largest

@@ -54,7 +54,7 @@ __device__ __forceinline__ bool box_raycast_hit(const GeomArgs& G, f3 p, f3 d, H
 __device__ __forceinline__ void overlay_hits(const JobView& A, f3 p, f3 d, bool& found, Hit& h)
 {
     for (uint32_t i = 0; i < A.n_ovl; i++) {
-        const Overlay ov = A.overlays[i];
+        const Overlay ov = glb(A.overlays)[i];
         float t0 = 0.0f, t1 = FLT_MAX;
         int axis = 3;
 #pragma unroll
@@ -116,28 +116,36 @@ __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Quad4T
         return mesh_raycast4_quad<COUNT, SK>(G, L, o, d, t, tri, fl, c);
 }
 
+// The hit record of a mesh walk's nearest triangle (normal oriented
+// against the ray), then the job's overlay boxes (WithPanels::raycast_hit).
+__device__ __forceinline__ bool mesh_hit(const JobView& A, f3 p, f3 d, bool found, float t,
+                                         uint32_t tri, uint32_t fl, Hit& h)
+{
+    if (found) {
+        float4 ti = *reinterpret_cast<const OMG_GLOBAL float4*>(glb(A.g.tris) + tri);
+        f3 n = mk(ti.x, ti.y, ti.z);
+        if (dot(n, d) > 0.0f) n = mul(n, -1.0f);
+        h.t = t;
+        h.n = n;
+        h.mat = __float_as_uint(ti.w);
+        h.id = tri;
+        h.flags = fl;
+    }
+    if (A.n_ovl) overlay_hits(A, p, d, found, h);
+    return found;
+}
+
 template <bool MESH, bool COUNT, int SK, class Trav>
 __device__ __forceinline__ bool raycast_hit(const JobView& A, const Trav& L, f3 p, f3 d,
                                             Hit& h, TravCount& c)
 {
-    bool found;
     if (MESH) {
         float t;
         uint32_t tri, fl;
-        found = mesh_raycast_any<COUNT, SK>(A.g, L, p, d, t, tri, fl, c);
-        if (found) {
-            float4 ti = *reinterpret_cast<const float4*>(A.g.tris + tri);
-            f3 n = mk(ti.x, ti.y, ti.z);
-            if (dot(n, d) > 0.0f) n = mul(n, -1.0f);
-            h.t = t;
-            h.n = n;
-            h.mat = __float_as_uint(ti.w);
-            h.id = tri;
-            h.flags = fl;
-        }
-    } else {
-        found = box_raycast_hit(A.g, p, d, h);
+        bool found = mesh_raycast_any<COUNT, SK>(A.g, L, p, d, t, tri, fl, c);
+        return mesh_hit(A, p, d, found, t, tri, fl, h);
     }
+    bool found = box_raycast_hit(A.g, p, d, h);
     if (A.n_ovl) overlay_hits(A, p, d, found, h);
     return found;
 }
@@ -221,7 +229,8 @@ __device__ __forceinline__ void ray_begin(const JobView& A, uint32_t src, uint32
 {
     S.src = src;
     S.ray = ray;
-    S.pos = mk(A.sources[3 * src], A.sources[3 * src + 1], A.sources[3 * src + 2]);
+    const OMG_GLOBAL float* sp = glb(A.sources) + 3 * src;
+    S.pos = mk(sp[0], sp[1], sp[2]);
     uint32_t w[4];
     philox4x32_10(ray, 0u, src, 0u, A.key0, A.key1, w);
     S.dir = unit_sphere(u24(w[0]), u24(w[1]));
@@ -242,19 +251,35 @@ __device__ __forceinline__ void ray_begin(const JobView& A, uint32_t src, uint32
 // band count (<= NBM; energies beyond it are zero and never scored).
 // LANES = 4 runs the raycast on a DPP quad (mesh_raycast_quad); the quad's
 // lanes then split the listeners between them.
+template <int NBM, int LANES>
+__device__ __forceinline__ bool ray_shade(const JobView& A, uint32_t q, uint32_t nb,
+                                          RayState<NBM>& S, bool has, const Hit& h,
+                                          uint32_t* hit_ids, float* hit_t);
+
 template <int NBM, bool MESH, bool COUNT, int SK, int LANES, class Trav>
 __device__ __forceinline__ bool ray_step(const JobView& A, const Trav& L, uint32_t q, uint32_t nb,
                                          RayState<NBM>& S, uint32_t* hit_ids, float* hit_t,
                                          TravCount& tc)
 {
-    const uint32_t bounce = S.bounce;
-    S.bounce = bounce + 1u;
     Hit h;
     bool has = raycast_hit<MESH, COUNT, SK>(A, L, S.pos, S.dir, h, tc);
+    return ray_shade<NBM, LANES>(A, q, nb, S, has, h, hit_ids, hit_t);
+}
+
+// The rest of a bounce once the segment's hit is known (has, h): listener
+// scoring, the per-bounce hit record, then the ray's fate.
+template <int NBM, int LANES>
+__device__ __forceinline__ bool ray_shade(const JobView& A, uint32_t q, uint32_t nb,
+                                          RayState<NBM>& S, bool has, const Hit& h,
+                                          uint32_t* hit_ids, float* hit_t)
+{
+    const uint32_t bounce = S.bounce;
+    S.bounce = bounce + 1u;
     float t_hit = has ? h.t : FLT_MAX;
     const size_t eg = (size_t)A.nbins * nb, dg = (size_t)A.nbins * 3;
     for (uint32_t li = q; li < A.n_lis; li += LANES) {
-        f3 Lp = mk(A.listeners[3 * li], A.listeners[3 * li + 1], A.listeners[3 * li + 2]);
+        const OMG_GLOBAL float* lp = glb(A.listeners) + 3 * li;
+        f3 Lp = mk(lp[0], lp[1], lp[2]);
         f3 to_l = sub(Lp, S.pos);
         float s = dot(to_l, S.dir);
         if (s > 0.0f && s < t_hit) {
@@ -268,22 +293,22 @@ __device__ __forceinline__ bool ray_step(const JobView& A, const Trav& L, uint32
                     float* od = A.dirs + pair * dg + (size_t)bin * 3;
 #pragma unroll
                     for (int b = 0; b < NBM; b++)
-                        if ((uint32_t)b < nb) unsafeAtomicAdd(ob + b, S.energy[b]);
+                        if ((uint32_t)b < nb) atomic_add_f32(ob + b, S.energy[b]);
                     float ed = S.energy[0];
 #pragma unroll
                     for (int b = 1; b < NBM; b++)
                         if ((uint32_t)b == A.dir_band) ed = S.energy[b];
-                    unsafeAtomicAdd(od + 0, -(S.dir.x * ed));
-                    unsafeAtomicAdd(od + 1, -(S.dir.y * ed));
-                    unsafeAtomicAdd(od + 2, -(S.dir.z * ed));
+                    atomic_add_f32(od + 0, -(S.dir.x * ed));
+                    atomic_add_f32(od + 1, -(S.dir.y * ed));
+                    atomic_add_f32(od + 2, -(S.dir.z * ed));
                 }
             }
         }
     }
     if (hit_ids && q == 0) {
         size_t slot = ((size_t)S.src * A.n_rays + S.ray) * A.max_bounces + bounce;
-        hit_ids[slot] = has ? h.id : 0xFFFFFFFFu;
-        hit_t[slot] = has ? h.t : 0.0f;
+        glb(hit_ids)[slot] = has ? h.id : 0xFFFFFFFFu;
+        glb(hit_t)[slot] = has ? h.t : 0.0f;
     }
     if (!has) {
         S.term = 0u;
@@ -297,7 +322,7 @@ __device__ __forceinline__ bool ray_step(const JobView& A, const Trav& L, uint32
         S.term = 1u;
         return false;
     }
-    const MatParams* M = A.mats + h.mat;
+    const OMG_GLOBAL MatParams* M = glb(A.mats) + h.mat;
     uint32_t w[4];
     philox4x32_10(S.ray, bounce + 1u, S.src, 0u, A.key0, A.key1, w);
     float p = M->p;
@@ -343,7 +368,7 @@ __device__ __forceinline__ void ray_end(const JobView& A, const RayState<NBM>& S
 {
     if (A.records && q == 0) {
         uint4 r = make_uint4(S.bounce, S.term, S.hash, S.rflags);
-        reinterpret_cast<uint4*>(A.records)[(size_t)S.src * A.n_rays + S.ray] = r;
+        reinterpret_cast<OMG_GLOBAL uint4*>(glb(A.records))[(size_t)S.src * A.n_rays + S.ray] = r;
     }
 }
 
@@ -536,6 +561,75 @@ void k_trace_batch_regen(BatchArgs B)
     if (COUNT && B.counters) flush_counters(B.counters, aabb_acc, tri_acc, iters_acc);
 }
 
+// Quad variant of the regenerating launch: a DPP quad runs each ray (the
+// 4-wide walk tests a node's four children on the quad's lanes, the binary
+// walk splits the pair tests; the lanes share the listeners), and a quad
+// whose ray has ended takes the next ray of its wave's item.  Ranks go to
+// the dead quads by the first lane of each quad: a quad's rank is the number
+// of dead quads before it in the wave.  With the 4-wide tree the whole
+// traversal stack (kDeepStack entries per quad) lives in LDS.
+template <int NBM, bool COUNT, int SK, int W>
+__global__ __launch_bounds__(256) void k_trace_batch_regen4(BatchArgs B)
+{
+    extern __shared__ float4 lds[];
+    typename TravOf<4, W>::type L;
+    init_trav(L, (lds_float4*)lds, 0u, SK);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t q = threadIdx.x & 3u;
+    const uint64_t before = (1ull << (lane & ~3u)) - 1ull;  // lanes ahead of this quad
+    RayState<NBM> S;
+    uint32_t live = 0, qi = 0;
+    uint32_t it_q = 0, it_src = 0, it_next = 0, it_end = 0, more = 1;
+    uint64_t aabb_acc = 0, tri_acc = 0, iters_acc = 0;
+    for (;;) {
+        // every lane of the wave is active here: lane 0 is the first active lane
+        uint64_t need = __ballot(live == 0u && q == 0u);
+        while (need != 0 && (it_next < it_end || more)) {
+            if (it_next == it_end) {
+                uint32_t it = 0;
+                if (lane == 0) it = atomicAdd(B.next_item, 1u);
+                it = __builtin_amdgcn_readfirstlane(it);
+                if (it >= B.n_items) {
+                    more = 0;
+                    break;
+                }
+                const BatchItem I = B.items[it];
+                it_q = I.query;
+                it_src = I.src;
+                it_next = I.ray_start;
+                it_end = I.ray_start + I.count;
+            }
+            const uint32_t avail = it_end - it_next;
+            const uint32_t rank = (uint32_t)__popcll(need & before);
+            if (live == 0u && rank < avail) {
+                ray_begin(B.jobs[it_q], it_src, it_next + rank, S);
+                qi = it_q;
+                live = 1u;
+            }
+            const uint32_t k = (uint32_t)__popcll(need);
+            it_next += k < avail ? k : avail;
+            need = __ballot(live == 0u && q == 0u);
+        }
+        if (__ballot(live != 0u) == 0) break;
+        if (live) {
+            const JobView& J = B.jobs[qi];
+            TravCount tc;
+            bool cont = ray_step<NBM, true, COUNT, SK, 4>(J, L, q, J.nbands, S, nullptr, nullptr, tc);
+            if (COUNT && q == 0) {
+                aabb_acc += tc.aabb;
+                tri_acc += tc.tri;
+            }
+            if (!cont) {
+                ray_end(J, S, q);
+                if (COUNT && q == 0) iters_acc += S.bounce;
+                live = 0u;
+            }
+        }
+    }
+    if (COUNT && B.counters) flush_counters(B.counters, aabb_acc, tri_acc, iters_acc);
+}
+
+
 __global__ void k_fill_zero(float* p, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -667,10 +761,22 @@ static hipError_t launch_batch_regen(const BatchArgs& B, uint32_t width, uint32_
     return hipGetLastError();
 }
 
+template <int NBM, bool COUNT, int SK, int W>
+static hipError_t launch_batch_regen4(const BatchArgs& B, uint32_t blocks, hipStream_t s)
+{
+    size_t lds = (size_t)SK * (kTraceBlock / 4) * 8;
+    hipLaunchKernelGGL((k_trace_batch_regen4<NBM, COUNT, SK, W>), dim3(blocks), dim3(kTraceBlock),
+                       lds, s, B);
+    return hipGetLastError();
+}
+
 template <int NBM, bool COUNT>
 static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint32_t blocks,
                                   hipStream_t s)
 {
+    if (sh.regen && sh.lanes == 4)
+        return sh.width == 4 ? launch_batch_regen4<NBM, COUNT, kDeepStack, 4>(B, blocks, s)
+                             : launch_batch_regen4<NBM, COUNT, 16, 2>(B, blocks, s);
     if (sh.regen) {
         switch (sh.stack_lds) {
         case 8: return launch_batch_regen<NBM, COUNT, 8>(B, sh.width, blocks, s);
@@ -679,9 +785,10 @@ static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint
         default: return hipErrorInvalidValue;
         }
     }
-    if (sh.width == 4)
-        return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 16, 4, 4>(B, blocks, s)
-                             : launch_batch_l<NBM, COUNT, 16, 1, 4>(B, blocks, s);
+    if (sh.width == 4 && sh.lanes == 4)
+        return sh.stack_lds == kDeepStack ? launch_batch_l<NBM, COUNT, kDeepStack, 4, 4>(B, blocks, s)
+                                          : launch_batch_l<NBM, COUNT, 16, 4, 4>(B, blocks, s);
+    if (sh.width == 4) return launch_batch_l<NBM, COUNT, 16, 1, 4>(B, blocks, s);
     if (sh.stack_lds >= 32)
         return sh.lanes == 4 ? launch_batch_l<NBM, COUNT, 32, 4, 2>(B, blocks, s)
                              : launch_batch_l<NBM, COUNT, 32, 1, 2>(B, blocks, s);
@@ -695,11 +802,7 @@ hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape
                               uint32_t n_cu, bool count, hipStream_t s, uint32_t max_blocks)
 {
     if (B.n_items == 0) return hipSuccess;
-    if (sh.lanes != 1 && sh.lanes != 4) return hipErrorInvalidValue;
-    if (sh.regen && sh.lanes != 1) return hipErrorInvalidValue;
-    if (sh.width != 2 && sh.width != 4) return hipErrorInvalidValue;
-    if (sh.width == 4 && !sh.regen && sh.stack_lds != 16) return hipErrorInvalidValue;
-    if (sh.waves_per_simd == 0 || sh.waves_per_simd > 8) return hipErrorInvalidValue;
+    if (!valid_batch_shape(sh)) return hipErrorInvalidValue;
     // 4 SIMDs per CU and 4 waves per 256-lane block: one block per CU per wave slot,
     // fewer when the batch has less work than that (concurrent batches then co-run)
     uint32_t blocks = n_cu * sh.waves_per_simd;

@@ -109,6 +109,23 @@ def test_batch_matches_plain_per_query(ctx, pool):
     b.close()
 
 
+def test_counting_batch_on_a_quad_shape_walks_the_binary_tree(ctx, pool):
+    """Counters are the reference walk's: a counting batch set to the
+    whole-stack quad shape runs its binary-tree counterpart."""
+    qs = _queries(pool)
+    _say("case counting batch, quad regen shape")
+    plain = [_plain(q) for q in qs]
+    b = H.Batch(ctx, qs, records=True, counters=True)
+    b.set_shape(4, 40, 6, 1, 4)
+    b.launch()
+    for i in range(len(qs)):
+        _same(b.result(i), plain[i], f"query {i}")
+    c = b.counters()
+    for k in ("aabb_tests", "tri_tests", "bounce_rays"):
+        assert c[k] == sum(p.counters[k] for p in plain), k
+    b.close()
+
+
 def test_batch_query_against_oracle(ctx, pool):
     """The 6-band query straight against the oracle's f64 re-accumulation."""
     qs = _queries(pool)
@@ -139,7 +156,7 @@ def test_batch_query_against_oracle(ctx, pool):
                          [(1, 16, 2, 0, 2), (1, 16, 8, 0, 2), (4, 16, 4, 0, 2), (1, 32, 4, 0, 2),
                           (4, 32, 6, 0, 2), (1, 16, 4, 1, 2), (1, 16, 8, 1, 2), (1, 8, 2, 1, 2),
                           (1, 12, 6, 1, 2), (1, 8, 8, 1, 4), (1, 16, 4, 1, 4), (4, 16, 4, 0, 4),
-                          (1, 16, 6, 0, 4)])
+                          (1, 16, 6, 0, 4), (4, 16, 6, 1, 2), (4, 40, 7, 1, 4), (4, 40, 4, 0, 4)])
 def test_batch_shapes_and_replays(ctx, pool, lanes, stack, wps, regen, width):
     qs = _queries(pool)
     _say(f"case batch shape lanes={lanes} stack_lds={stack} waves/simd={wps} regen={regen} "
@@ -161,10 +178,10 @@ def test_batch_tune(ctx, pool):
     _say("case batch autotune")
     b = H.Batch(ctx, qs, records=True)
     shape = b.tune(stride=2)
-    assert shape["lanes"] in (1, 4) and shape["stack_lds"] in (8, 12, 16, 32)
+    assert shape["lanes"] in (1, 4) and shape["stack_lds"] in (8, 12, 16, 32, 40)
     assert shape["width"] in (2, 4)
     assert 1 <= shape["waves_per_simd"] <= 8 and shape["ms"] > 0
-    assert shape["regen"] in (0, 1) and not (shape["regen"] and shape["lanes"] != 1)
+    assert shape["regen"] in (0, 1)
     b.launch()
     for i, q in enumerate(qs):
         _same(b.result(i), _plain(q), f"tuned query {i}", wide=shape["width"] == 4)
@@ -194,7 +211,11 @@ def test_batch_refusals(ctx, pool):
     with pytest.raises(H.OmgError):
         b.set_shape(2, 16, 4)
     with pytest.raises(H.OmgError):
-        b.set_shape(4, 16, 4, regen=1)   # regeneration runs one lane per ray
+        b.set_shape(4, 8, 4, regen=1)    # a regenerating quad keeps 16 entries (binary tree)
+    with pytest.raises(H.OmgError):
+        b.set_shape(4, 40, 4, regen=1, width=2)   # ... or 40 (4-wide tree only)
+    with pytest.raises(H.OmgError):
+        b.set_shape(1, 40, 4, regen=1, width=4)   # the whole-stack walk runs on a quad
     with pytest.raises(H.OmgError):
         b.set_shape(1, 32, 4, regen=1)   # and keeps at most 16 stack entries in LDS
     with pytest.raises(H.OmgError):

@@ -57,18 +57,20 @@ def _mix(world, n, rng):
     return out
 
 
-@pytest.mark.parametrize("latency_rays", [0, 1 << 20])
-def test_request_alone_and_in_mix_match_plain(ctx, world, latency_rays):
-    """latency_rays = 0: every batch runs the regenerating walk; 1 << 20:
-    these small batches run a DPP quad per ray over 16-ray items."""
+@pytest.mark.parametrize("latency_rays,shape", [(0, None), (1 << 20, None), (0, "quad")])
+def test_request_alone_and_in_mix_match_plain(ctx, world, latency_rays, shape):
+    """latency_rays = 0: every batch runs the regenerating walk (one lane per
+    ray, or a quad per ray with the whole stack in LDS); 1 << 20: these
+    small batches run a DPP quad per ray over 16-ray items."""
     sa, sb = world["scenes"]
     pts = world["points"][1]
     probe = dict(scene=sb, params=_params(2500, 4242), sources=pts[[0, 3, 7]],
                  listeners=pts[[5, 9]], dome=True, keep=True, records=True)
     dp = H.dome_params(3, n_fan=512, horizon=500.0)
+    sh = dict(lanes=4, stack_lds=40, waves_per_simd=7, regen=1, width=4) if shape else None
     srv = H.Server(ctx, [sa, sb], hbm_budget=1 << 34, slots=3, max_batch=16, dome=dp,
-                   latency_rays=latency_rays)
-    _say(f"case serve (latency_rays={latency_rays}): probe request alone")
+                   latency_rays=latency_rays, shape=sh)
+    _say(f"case serve (latency_rays={latency_rays}, shape {shape}): probe request alone")
     t = srv.submit(probe["scene"], probe["params"], probe["sources"], probe["listeners"],
                    dome=True, keep=True, records=True)
     alone = srv.wait(t)
