@@ -26,11 +26,11 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per k_trace launch of this workload, measured with rocprofv3 PMC
 # (FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, scales from a 256 MiB copy
 # kernel), profiles/k_trace_city_block_50k.md
-K_TRACE_TRAFFIC_BYTES = 18.44e6
+K_TRACE_TRAFFIC_BYTES = 18.39e6
 # the same for the k_trace_batch_regen launch of the mixed 256-query
-# workload in the shape the autotuner usually picks (1,8,8,1,4),
+# workload in the shape the autotuner picks (1,12,6,1,4),
 # profiles/k_trace_batch_regen_mixed_256q.md
-BATCH_TRAFFIC_BYTES = 230.4e9
+BATCH_TRAFFIC_BYTES = 16.50e9
 NODE_BYTES = 32           # bytes per AABB test (one child box, crates/omg-gpu/src/layout.rs:96-107)
 PRIM_BYTES = 48           # bytes per triangle test (layout.rs:109-122)
 
