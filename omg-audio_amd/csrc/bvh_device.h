@@ -67,7 +67,11 @@ __device__ __forceinline__ bool ray_aabb(const float bmin[3], const float bmax[3
     return !(t0 > t1);
 }
 
-// Mesh::ray_tri_packed (mesh.rs:437-456): two-sided Moller-Trumbore.
+// Mesh::ray_tri_packed (mesh.rs:437-456): two-sided Moller-Trumbore.  The
+// reference returns at the first failed test; here every quantity is
+// computed with the same operations and the four tests are combined at the
+// end, which gives the same answer and the same t, u, v without branches
+// (t_out, u_out, v_out are written either way).
 __device__ __forceinline__ bool ray_tri(const Prim& p, f3 o, f3 d, float& t_out, float& u_out,
                                         float& v_out)
 {
@@ -75,20 +79,18 @@ __device__ __forceinline__ bool ray_tri(const Prim& p, f3 o, f3 d, float& t_out,
     f3 e2 = mk(p.e2[0], p.e2[1], p.e2[2]);
     f3 pv = cross(d, e2);
     float det = dot(e1, pv);
-    if (fabsf(det) < 1e-9f) return false;
     float inv = 1.0f / det;
     f3 s = sub(o, mk(p.a[0], p.a[1], p.a[2]));
     float u = dot(s, pv) * inv;
-    if (!(u >= -1e-6f && u <= 1.0f + 1e-6f)) return false;
     f3 q = cross(s, e1);
     float v = dot(d, q) * inv;
-    if (v < -1e-6f || u + v > 1.0f + 1e-6f) return false;
     float t = dot(e2, q) * inv;
-    if (!(t > 1e-5f)) return false;
+    const bool ok = !(fabsf(det) < 1e-9f) & (u >= -1e-6f) & (u <= 1.0f + 1e-6f) &
+                    !((v < -1e-6f) | (u + v > 1.0f + 1e-6f)) & (t > 1e-5f);
     t_out = t;
     u_out = u;
     v_out = v;
-    return true;
+    return ok;
 }
 
 __device__ __forceinline__ Prim load_prim(const Prim* prims, uint32_t k)
@@ -539,12 +541,14 @@ __device__ __forceinline__ int order_children(const float4 lo[4], const float4 h
 }
 
 // Resumable state of the 4-wide walk of one lane: the node in hand, its
-// entry distance, the stack depth and the nearest hit so far.
+// entry distance, the stack depth, the primitives [pk, pend) of the leaf
+// being tested and the nearest hit so far.
 struct Walk4 {
     f3 inv;
     uint32_t cur;
     float cur_t;
     int sp;
+    uint32_t pk, pend;
     BestHit H;
 };
 
@@ -555,28 +559,29 @@ __device__ __forceinline__ bool walk4_begin(const GeomArgs& G, Walk4& W, f3 d)
     W.cur = G.root4;
     W.cur_t = 0.0f;
     W.sp = 0;
+    W.pk = 0;
+    W.pend = 0;
     W.H = BestHit();
     return !G.empty;
 }
 
-// One step of the nearest-hit walk over the 4-wide nodes (one node tested,
-// or one leaf's primitives, or one pop); false once the walk has ended.
-// The stack's top SK entries live in LDS, deeper ones in ovf.
+// One step of the nearest-hit walk over the 4-wide nodes: one node tested,
+// or one primitive of a leaf, then a pop when the lane has nothing in
+// hand; false once the walk has ended.  A leaf's primitives take one step
+// each (in order), so that the lanes of a wave that are in leaves and those
+// in inner nodes share every step instead of the whole wave waiting for its
+// largest leaf.  Nodes, primitives and their order are those of the walk
+// that tests a leaf at once.  The stack's top SK entries live in LDS,
+// deeper ones in ovf.
 template <bool COUNT, int SK>
 __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L, Walk4& W, f3 o,
                                            f3 d, uint32_t (&ovf_ref)[kStackCap - SK],
                                            float (&ovf_t)[kStackCap - SK], TravCount& cnt)
 {
-    if (!(W.H.found() && W.cur_t >= W.H.best)) {
+    if (W.pk == W.pend && !(W.H.found() && W.cur_t >= W.H.best)) {
         if (W.cur & kLeafBit) {
-            uint32_t start = W.cur & kLeafStartMask;
-            uint32_t cnt_p = (W.cur >> kLeafCountShift) & 7u;
-            for (uint32_t k = start; k < start + cnt_p; k++) {
-                Prim p = load_prim(G.prims, k);
-                if (COUNT) cnt.tri++;
-                float t, u, v;
-                if (ray_tri(p, o, d, t, u, v)) W.H.offer(t, p.tri, u, v);
-            }
+            W.pk = W.cur & kLeafStartMask;
+            W.pend = W.pk + ((W.cur >> kLeafCountShift) & 7u);
         } else {
             float4 lo[4], hi[4];
             if (W.cur < L.nc) {
@@ -617,6 +622,14 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
                 return true;
             }
         }
+    }
+    if (W.pk < W.pend) {
+        Prim p = load_prim(G.prims, W.pk);
+        if (COUNT) cnt.tri++;
+        float t, u, v;
+        if (ray_tri(p, o, d, t, u, v)) W.H.offer(t, p.tri, u, v);
+        W.pk++;
+        if (W.pk < W.pend) return true;
     }
     if (W.sp == 0) return false;
     W.sp--;
