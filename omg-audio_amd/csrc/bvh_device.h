@@ -119,18 +119,17 @@ struct BestHit {
     uint32_t btri = kNoHit, tie = 0;
 
     __device__ __forceinline__ bool found() const { return btri != kNoHit; }
-    // strict `t < best`; an exact-t tie with another triangle is flagged
-    __device__ __forceinline__ void offer(float t, uint32_t tri, float u, float v)
+    // strict `t < best`; an exact-t tie with another triangle is flagged.
+    // Offered only when `take`; written with selects, not branches.
+    __device__ __forceinline__ void offer_if(bool take, float t, uint32_t tri, float u, float v)
     {
-        if (!found() || t < best) {
-            best = t;
-            btri = tri;
-            bu = u;
-            bv = v;
-            tie = 0;
-        } else if (t == best && tri != btri) {
-            tie = 1;
-        }
+        const bool better = take & (!found() | (t < best));
+        const bool same = take & !better & (t == best) & (tri != btri);
+        best = better ? t : best;
+        btri = better ? tri : btri;
+        bu = better ? u : bu;
+        bv = better ? v : bv;
+        tie = better ? 0u : (same ? 1u : tie);
     }
     // flags: bit0 winning hit within 1e-6 barycentric of an edge, bit1 exact-t tie
     __device__ __forceinline__ bool result(float& t_out, uint32_t& tri_out, uint32_t& flags) const
@@ -171,7 +170,7 @@ __device__ __forceinline__ bool mesh_raycast(const GeomArgs& G, f3 o, f3 d, floa
                 Prim p = load_prim(G.prims, k);
                 if (COUNT) cnt.tri++;
                 float t, u, v;
-                if (ray_tri(p, o, d, t, u, v)) H.offer(t, p.tri, u, v);
+                H.offer_if(ray_tri(p, o, d, t, u, v), t, p.tri, u, v);
             }
         } else {
             const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + ref);
@@ -276,7 +275,7 @@ __device__ __forceinline__ bool mesh_raycast_lds(const GeomArgs& G, const LdsTra
                     Prim p = load_prim(G.prims, k);
                     if (COUNT) cnt.tri++;
                     float t, u, v;
-                    if (ray_tri(p, o, d, t, u, v)) H.offer(t, p.tri, u, v);
+                    H.offer_if(ray_tri(p, o, d, t, u, v), t, p.tri, u, v);
                 }
             } else {
                 float4 n0, n1, n2, n3;
@@ -361,7 +360,7 @@ __device__ __forceinline__ void fold_prim(uint32_t cnt_p, uint32_t hit_q, float 
     float t = qbf<J>(t_q);
     uint32_t tri = qb<J>(tri_q);
     float u = qbf<J>(u_q), v = qbf<J>(v_q);
-    if ((uint32_t)J < cnt_p && h) H.offer(t, tri, u, v);
+    H.offer_if(((uint32_t)J < cnt_p) & (h != 0u), t, tri, u, v);
 }
 
 template <bool COUNT, int SK>
@@ -627,7 +626,7 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
         Prim p = load_prim(G.prims, W.pk);
         if (COUNT) cnt.tri++;
         float t, u, v;
-        if (ray_tri(p, o, d, t, u, v)) W.H.offer(t, p.tri, u, v);
+        W.H.offer_if(ray_tri(p, o, d, t, u, v), t, p.tri, u, v);
         W.pk++;
         if (W.pk < W.pend) return true;
     }
@@ -820,10 +819,10 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                     // rank of this lane's child among the quad's hits
                     const float t0 = qbf<0>(te), t1 = qbf<1>(te), t2 = qbf<2>(te), t3 = qbf<3>(te);
                     int rank = 0;
-                    rank += ((hm & 1u) && (t0 < te || (t0 == te && 0u < q))) ? 1 : 0;
-                    rank += ((hm & 2u) && (t1 < te || (t1 == te && 1u < q))) ? 1 : 0;
-                    rank += ((hm & 4u) && (t2 < te || (t2 == te && 2u < q))) ? 1 : 0;
-                    rank += ((hm & 8u) && (t3 < te || (t3 == te && 3u < q))) ? 1 : 0;
+                    rank += ((hm & 1u) != 0u) & ((t0 < te) | ((t0 == te) & (0u < q))) ? 1 : 0;
+                    rank += ((hm & 2u) != 0u) & ((t1 < te) | ((t1 == te) & (1u < q))) ? 1 : 0;
+                    rank += ((hm & 4u) != 0u) & ((t2 < te) | ((t2 == te) & (2u < q))) ? 1 : 0;
+                    rank += ((hm & 8u) != 0u) & ((t3 < te) | ((t3 == te) & (3u < q))) ? 1 : 0;
                     if (h && rank > 0) {
                         const int pos = sp + nhit - 1 - rank;
                         L.st_ref[pos * L.nq + L.quad] = ref;
