@@ -30,7 +30,7 @@ K_TRACE_TRAFFIC_BYTES = 18.39e6
 # the same for the k_trace_batch_regen launch of the mixed 256-query
 # workload in the shape the autotuner picks (1,12,6,1,4),
 # profiles/k_trace_batch_regen_mixed_256q.md
-BATCH_TRAFFIC_BYTES = 16.50e9
+BATCH_TRAFFIC_BYTES = 17.38e9
 NODE_BYTES = 32           # bytes per AABB test (one child box, crates/omg-gpu/src/layout.rs:96-107)
 PRIM_BYTES = 48           # bytes per triangle test (layout.rs:109-122)
 
@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-stress", action="store_true", help="skip the 2M-triangle town")
     ap.add_argument("--batch-shape", default="",
-                    help="lanes,stack_lds,waves_per_simd,regen,width for the batched launch "
+                    help="lanes,stack_lds,waves_per_simd,regen,width for the batched launches "
                          "instead of the autotuner's choice (reproducible profiles)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run a plain elementwise copy of 256 MiB "
@@ -228,7 +228,7 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
             "peak_committed_bytes": st["peak_committed_bytes"]}
 
 
-def stress(ctx, dev, launches=2):
+def stress(ctx, dev, launches=2, fixed_shape=None):
     """Large-scene stress: the ~2M-triangle town (400 buildings with
     storey slabs, partitions and apertures), 64 sources x 1M rays x 6
     bands scored at 4 listeners (1 ms x 2 s), 128 bounces, floor 1e-7.  The
@@ -270,7 +270,10 @@ def stress(ctx, dev, launches=2):
     got = d_bins.cpu().numpy()
     rel = float(np.max(np.abs(got - counted.bins)) / max(float(np.max(np.abs(counted.bins))), 1e-30))
     b = H.Batch(ctx, [H.QuerySpec(scene, params, src, lis)])
-    b.tune(stride=64)
+    if fixed_shape:
+        b.set_shape(*fixed_shape)
+    else:
+        b.tune(stride=64)
     b.launch()
     b.sync()
     batch_s = []
@@ -428,13 +431,13 @@ def main():
             torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
         del x, y
+    fixed = [int(x) for x in args.batch_shape.split(",")] if args.batch_shape else None
     if not args.no_batch:
-        fixed = [int(x) for x in args.batch_shape.split(",")] if args.batch_shape else None
         out["batched_vs_plain"] = batched_vs_plain(ctx, fixed_shape=fixed)
     if not args.no_serve:
         out["serving"] = serving(ctx)
     if not args.no_stress:
-        out["stress"] = stress(ctx, dev)
+        out["stress"] = stress(ctx, dev, fixed_shape=fixed)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -267,7 +267,8 @@ int omg_batch_size(omg_batch *batch, uint64_t *rays, uint32_t *items);
 int omg_batch_get_shape(omg_batch *batch, omg_batch_shape *shape);
 int omg_batch_set_shape(omg_batch *batch, const omg_batch_shape *shape);
 /* Autotuner: times every launch shape on a strided sample of the batch's
- * work items (every `stride`-th), keeps the fastest, re-captures the graph
+ * work items (every `stride`-th; the stride is lowered so that the sample
+ * keeps at least 256 items per compute unit), keeps the fastest, re-captures the graph
  * and caches the choice on the context for batches of the same class
  * (widest band count, total rays and mean BVH size, each to a power of 2).
  * out may be NULL.  Outputs of the sampled launches are overwritten by

@@ -397,6 +397,11 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
     if (!b) return fail(OMG_ERR_INVALID, "bad argument");
     if (stride == 0) stride = 1;
     omg_ctx* c = b->ctx;
+    // a sample too small to keep every lane busy for several rays judges
+    // occupancy by the launch's tail: keep at least 256 items per CU
+    const uint64_t min_items = (uint64_t)c->n_cu * 256u;
+    const uint64_t cap = std::max<uint64_t>(1, b->items_host.size() / min_items);
+    if (stride > cap) stride = (uint32_t)cap;
     OMG_HIP_CHECK(hipSetDevice(c->device));
     int rc = omg_batch_sync(b);
     if (rc) return rc;
