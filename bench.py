@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-stress", action="store_true", help="skip the 2M-triangle town")
+    ap.add_argument("--no-late", action="store_true", help="skip the late-field state workload")
     ap.add_argument("--batch-shape", default="",
                     help="lanes,stack_lds,waves_per_simd,regen,width for the batched launches "
                          "instead of the autotuner's choice (reproducible profiles)")
@@ -226,6 +227,48 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
             "max_qps_with_p99_under_10ms": max(held) if held else None,
             "batches": st["batches"], "mean_batch": st["mean_batch"],
             "peak_committed_bytes": st["peak_committed_bytes"]}
+
+
+def late_field(ctx, dev, n_slots=4096, reps=5):
+    """Late-field state of 4096 listener slots (2 s x 1 ms x 3 bands): one
+    omg_late_field_update call folds a fresh device echogram into every
+    slot (EMA), reduces every slot (f64 decay integral, late energy and
+    direction) and returns the reverb parameters to the host; the timed
+    region is that C call (update table built once)."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import omg_hip as H
+    from omg_hip import _lib
+    lf = H.LateField(ctx, n_slots, nbands=3, bin_dt=0.001, max_time=2.0)
+    n = lf.nbins
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    t = (torch.arange(n, device=dev, dtype=torch.float32) + 0.5) * 1e-3
+    tail = torch.exp(-t[:, None] / torch.tensor([0.1, 0.15, 0.2], device=dev))
+    cur_b = (tail[None] * torch.rand((n_slots, n, 3), generator=g, device=dev)).contiguous()
+    cur_d = (torch.randn((n_slots, n, 3), generator=g, device=dev) * cur_b[..., 1:2]).contiguous()
+    torch.cuda.synchronize()
+    ups = (_lib.LateUpdate * n_slots)()
+    for s in range(n_slots):
+        ups[s].slot = s
+        ups[s].traced = 1
+        ups[s].cur_bins = cur_b[s].data_ptr()
+        ups[s].cur_dirs = cur_d[s].data_ptr()
+        ups[s].receiver[0] = 3.0
+    out = (_lib.LateOut * n_slots)()
+    L = _lib.lib()
+    _lib.check(L.omg_late_field_update(lf.h, ups, n_slots, out, None))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _lib.check(L.omg_late_field_update(lf.h, ups, n_slots, out, None))
+        ts.append(time.perf_counter() - t0)
+    lf.close()
+    dt = float(np.median(ts))
+    return {"workload": f"{n_slots} slots x {n} bins x 3 bands, every slot traced each call",
+            "ms_per_call": dt * 1e3, "slot_updates_per_s": n_slots / dt,
+            "rt60_slot0": [float(x) for x in out[0].rt60[:3]]}
 
 
 def stress(ctx, dev, launches=2, fixed_shape=None):
@@ -438,6 +481,8 @@ def main():
         out["serving"] = serving(ctx)
     if not args.no_stress:
         out["stress"] = stress(ctx, dev, fixed_shape=fixed)
+    if not args.no_late:
+        out["late_field"] = late_field(ctx, dev)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb

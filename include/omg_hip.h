@@ -478,6 +478,89 @@ int omg_server_stats_get(omg_server *srv, omg_server_stats *st);
 /* Drains the in-flight work, stops the thread, frees the arenas. */
 void omg_server_destroy(omg_server *srv);
 
+/* ------------------------------------------------------------ late-field state (sim side)
+ *
+ * The consumer of the tracer inside the reference's Sim: TraceGate
+ * (crates/omg-scene/src/sim.rs:38-76) decides whether a slot's late field
+ * is re-traced this tick; a fresh echogram is folded into the slot's
+ * running average with Echogram::ema (crates/omg-core/src/tracer.rs:46-57,
+ * alpha 1.0 on a slot's first update, 0.3 after, sim.rs:330-333 and
+ * :374-377); the average yields ReverbParams via estimate_reverb
+ * (tracer.rs:205-257), the late direction and anisotropy via
+ * late_direction(0.08) (tracer.rs:62-77), and the level is scaled for
+ * the source-receiver distance change since the last trace (sim.rs:382-395).
+ * Sim::reverb_world (sim.rs:349-397) is that sequence.  Here the averages of
+ * all slots stay resident on the device, one call updates any number of
+ * slots, and only the parameters come back to the host. */
+typedef struct omg_late_field omg_late_field;
+
+/* TraceGate (sim.rs:38-76); caller-owned state, no device needed. */
+typedef struct omg_trace_gate {
+    float last_src[3];
+    float last_lis[3];
+    float last_energy[OMG_MAX_BANDS];
+    uint32_t age;
+} omg_trace_gate;
+/* TraceGate::new(phase) (sim.rs:46-53) */
+void omg_trace_gate_init(omg_trace_gate *g, uint32_t phase);
+/* TraceGate::should_trace (sim.rs:55-75): *fire = 1 when the slot should be
+ * re-traced (motion > 0.25 m, a band's energy moved by > 10 %, or max_age
+ * ticks since the last trace - the quality tier's gate_max_age,
+ * quality.rs:133-139, :177-179). */
+int omg_trace_gate_step(omg_trace_gate *g, const float src[3], const float lis[3],
+                        const float *energy, uint32_t nbands, uint32_t max_age, int *fire);
+
+/* n_slots averages of [nbins][nbands] bins + [nbins][3] dirs, zero at start;
+ * slot s's gate starts at phase s % 8 (Sim::new, sim.rs:99-103).
+ * 2 <= nbands <= OMG_MAX_BANDS (late_direction reads band 1). */
+int omg_late_field_create(omg_ctx *ctx, uint32_t n_slots, uint32_t nbins, uint32_t nbands,
+                          float bin_dt, omg_late_field **out);
+/* The slot's TraceGate::should_trace. */
+int omg_late_field_gate(omg_late_field *lf, uint32_t slot, const float src[3], const float lis[3],
+                        const float *energy, uint32_t max_age, int *fire);
+
+typedef struct omg_late_update {
+    uint32_t slot;
+    uint32_t traced;          /* 1: fold cur_bins/cur_dirs into the slot's average */
+    const float *cur_bins;    /* device [nbins][nbands] (a trace's echogram), or NULL */
+    const float *cur_dirs;    /* device [nbins][3], or NULL */
+    float src[3];             /* source and receiver of this tick */
+    float receiver[3];
+} omg_late_update;
+
+typedef struct omg_late_out {
+    float rt60[OMG_MAX_BANDS];   /* ReverbParams (params.rs:19-29) */
+    float level[OMG_MAX_BANDS];  /* after the distance compensation */
+    float dir[3];                /* late_direction(0.08) */
+    float aniso;
+    uint32_t version;            /* the slot's update count after this call */
+    uint32_t traced;
+} omg_late_out;
+
+/* Updates n distinct slots (sim.rs:365-397 after the trace decision):
+ * EMA for the traced ones, then reverb parameters, late direction and the
+ * distance compensation for every one.  The fresh echograms must be
+ * complete on `stream` (a hipStream_t; NULL = the field's stream, then
+ * the caller has synchronized them).  Blocks until out[] is written. */
+int omg_late_field_update(omg_late_field *lf, const omg_late_update *ups, uint32_t n,
+                          omg_late_out *out, void *stream);
+typedef struct omg_late_trace {
+    uint32_t slot;
+    omg_trace_params params;  /* nbands and nbins(max_time, bin_dt) must be the field's */
+    float src[3];
+    float receiver[3];
+} omg_late_trace;
+/* Traces each job's late field on the scene (one source, one listener:
+ * WorldLateBackend::trace_world, late.rs:81-91) into the field's own device
+ * staging, then updates the slots as omg_late_field_update with traced = 1
+ * (sim.rs:366-377).  Slots must be distinct.  Blocks until out[] is written. */
+int omg_late_field_trace_world(omg_late_field *lf, omg_scene *scene, const omg_late_trace *jobs,
+                               uint32_t n, const omg_overlay *overlays, uint32_t n_ovl,
+                               omg_late_out *out);
+/* Copies a slot's average to host buffers ([nbins][nbands], [nbins][3]). */
+int omg_late_field_read(omg_late_field *lf, uint32_t slot, float *bins, float *dirs);
+void omg_late_field_destroy(omg_late_field *lf);
+
 #ifdef __cplusplus
 }
 #endif

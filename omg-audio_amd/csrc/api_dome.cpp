@@ -67,6 +67,10 @@ extern "C" int omg_dome_sample(omg_scene* s, const omg_dome_params* p, const flo
     if (n_eyes == 0) return OMG_OK;
     omg_ctx* c = s->ctx;
     OMG_HIP_CHECK(hipSetDevice(c->device));
+    // the panels go to job_ovl, which an in-flight trace may read as overlays
+    rc = quiesce(c);
+    if (rc) return rc;
+    c->tables_valid = false;
     std::vector<float> fan;
     std::vector<uint32_t> fan_bin;
     make_fan(p->n_fan, fan, fan_bin);
@@ -137,6 +141,8 @@ extern "C" int omg_dome_sample_async(omg_scene* s, const omg_dome_params* p, con
     OMG_HIP_CHECK(hipSetDevice(c->device));
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     if (c->fan_n != p->n_fan || !c->dome_status.ptr) {
+        rc = quiesce(c);   // an earlier dome launch may still read the fan
+        if (rc) return rc;
         std::vector<float> fan;
         std::vector<uint32_t> fan_bin;
         make_fan(p->n_fan, fan, fan_bin);
@@ -151,7 +157,8 @@ extern "C" int omg_dome_sample_async(omg_scene* s, const omg_dome_params* p, con
         c->fan_n = p->n_fan;
     }
     if (n_panels) {
-        rc = upload(c->dome_panels, panels, n_panels * sizeof(omg_dome_panel), c->stream);
+        rc = quiesce(c);   // an earlier dome launch may still read the panels
+        if (!rc) rc = upload(c->dome_panels, panels, n_panels * sizeof(omg_dome_panel), c->stream);
         if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "panels");
         if (rc) return rc;
     } else {
@@ -177,6 +184,7 @@ extern "C" int omg_dome_sample_async(omg_scene* s, const omg_dome_params* p, con
     A.dsum = d_dsum;
     A.counters = nullptr;
     A.status = c->dome_status.as<uint32_t>();
+    c->async_inflight = true;
     OMG_HIP_CHECK(launch_dome(A, false, st));
     return OMG_OK;
 }

@@ -62,6 +62,8 @@ extern "C" int omg_discover(omg_scene* s, const float listener[3], uint32_t n_ra
     if (n_rays == 0) return OMG_OK;
     omg_ctx* c = s->ctx;
     OMG_HIP_CHECK(hipSetDevice(c->device));
+    int qrc = quiesce(c);   // the context's staging may still be read by *_async work
+    if (qrc) return qrc;
     // the rotating fan's jitter is one sequential xorshift64* stream (pt_mesh.rs:386-392)
     std::vector<float> jitter(n_rays);
     uint64_t x = (0xC6Bull ^ (uint64_t)rot) | 1ull;
@@ -125,7 +127,11 @@ extern "C" int omg_solve_batch(omg_scene* s, const omg_surface* surfaces, uint32
     if (total >= (1ull << 31)) return fail(OMG_ERR_CAP, "too many (source, chain) pairs");
     omg_ctx* c = s->ctx;
     OMG_HIP_CHECK(hipSetDevice(c->device));
-    int rc = upload(c->aux0, surfaces, n_surf * sizeof(omg_surface), c->stream);
+    // the extras go to job_ovl, which an in-flight trace may read as overlays
+    int rc = quiesce(c);
+    if (rc) return rc;
+    c->tables_valid = false;
+    rc = upload(c->aux0, surfaces, n_surf * sizeof(omg_surface), c->stream);
     if (!rc) rc = upload(c->aux1, sources, (size_t)n_src * 3 * sizeof(float), c->stream);
     if (!rc) rc = upload(c->aux2, chains, (size_t)n_chains * 3 * sizeof(uint16_t), c->stream);
     if (!rc) rc = upload(c->aux3, orders, n_chains, c->stream);

@@ -296,6 +296,14 @@ extern "C" void omg_scene_destroy(omg_scene* s)
     ctx_release(c);
 }
 
+int omg::quiesce(omg_ctx* c)
+{
+    if (!c->async_inflight) return OMG_OK;
+    OMG_HIP_CHECK(hipDeviceSynchronize());
+    c->async_inflight = false;
+    return OMG_OK;
+}
+
 void omg::ctx_release(omg_ctx* c)
 {
     c->live_scenes--;
@@ -375,11 +383,22 @@ int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatPara
     std::vector<MatParams> mats;
     std::vector<Overlay> ov;
     job_tables(base_mats, ovl, n_ovl, nbands, mats, ov);
-    int rc = upload(c->job_mats, mats.data(), mats.size() * sizeof(MatParams), c->stream);
-    if (!rc) rc = upload(c->job_ovl, ov.data(), ov.size() * sizeof(Overlay), c->stream);
-    // the staging vectors die with this frame: wait for the copies to land
-    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "job upload");
-    if (rc) return rc;
+    std::vector<uint8_t> key(mats.size() * sizeof(MatParams) + ov.size() * sizeof(Overlay) + 8);
+    uint32_t sizes[2] = {(uint32_t)mats.size(), (uint32_t)ov.size()};
+    std::memcpy(key.data(), sizes, 8);
+    if (!mats.empty()) std::memcpy(key.data() + 8, mats.data(), mats.size() * sizeof(MatParams));
+    if (!ov.empty())
+        std::memcpy(key.data() + 8 + mats.size() * sizeof(MatParams), ov.data(), ov.size() * sizeof(Overlay));
+    if (!(c->tables_valid && key == c->tables_key)) {
+        int rc = quiesce(c);
+        if (!rc) rc = upload(c->job_mats, mats.data(), mats.size() * sizeof(MatParams), c->stream);
+        if (!rc) rc = upload(c->job_ovl, ov.data(), ov.size() * sizeof(Overlay), c->stream);
+        // the staging vectors die with this frame: wait for the copies to land
+        if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "job upload");
+        c->tables_valid = rc == OMG_OK;
+        if (rc) return rc;
+        c->tables_key.swap(key);
+    }
     A.mats = c->job_mats.as<MatParams>();
     A.overlays = c->job_ovl.as<Overlay>();
     job_scalars(p, nbands, n_ovl, A);
@@ -604,6 +623,7 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
     A.ray_count = p->n_rays * n_src;
     OMG_HIP_CHECK(launch_fill_zero(d_bins, nb_f, st));
     OMG_HIP_CHECK(launch_fill_zero(d_dirs, nd_f, st));
+    c->async_inflight = true;
     OMG_HIP_CHECK(launch_trace(A, true, false, st));
     return OMG_OK;
 }

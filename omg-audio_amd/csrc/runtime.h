@@ -85,6 +85,13 @@ struct omg_ctx {
     uint64_t quad_below = 1ull << 20;   // launches with fewer rays trace a quad per ray
     uint32_t n_cu = 0;                  // compute units (persistent launches size their grid to it)
     bool closing = false;
+    // the *_async entry points leave kernels in flight that read the staged
+    // job tables (job_mats, job_ovl) and dome fan / panels; staging the same
+    // bytes again is skipped, and anything that overwrites a buffer such a
+    // kernel may read first waits for the device (omg::quiesce)
+    bool async_inflight = false;
+    bool tables_valid = false;
+    std::vector<uint8_t> tables_key;
 };
 
 struct omg_scene {
@@ -121,4 +128,6 @@ void job_scalars(const omg_trace_params* p, uint32_t nbands, uint32_t n_ovl, Job
 // drops one scene/batch/server reference; tears the context down after omg_shutdown
 void ctx_release(omg_ctx* c);
 void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins);
+// waits for in-flight *_async work before shared staging is overwritten
+int quiesce(omg_ctx* c);
 }  // namespace omg

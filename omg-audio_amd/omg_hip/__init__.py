@@ -9,6 +9,8 @@ The host-side mirror of the reference interface over the C ABI
 - ``GpuWorldDiscovery.discover`` / ``GpuWorldSolve.solve_batch`` =
   ``WorldDiscovery`` / ``WorldSolve`` (crates/omg-scene/src/early_world.rs:24-52)
 - ``estimate_reverb`` / ``late_direction`` (crates/omg-core/src/tracer.rs:205-257, :62-77)
+- ``LateField.reverb_world`` / ``TraceGate`` = the late path of ``Sim::reverb_world``
+  (crates/omg-scene/src/sim.rs:38-76, :349-397), averages resident on the device
 
 Like the reference backends, a refused call raises ``OmgError`` (the
 reference returns ``false``/``None``); construction raises when no GPU is
@@ -85,6 +87,7 @@ class Context:
         h = C.c_void_p()
         check(lib().omg_init(device, LAYOUT_VERSION, C.byref(h)))
         self.h = h
+        self.device = device
         buf = C.create_string_buffer(128)
         check(lib().omg_device_name(self.h, buf, 128))
         self.device_name = buf.value.decode()
@@ -388,5 +391,6 @@ from .batch import Batch, QuerySpec  # noqa: E402
 
 __all__ += ["Batch", "QuerySpec"]
 from .serve import Server  # noqa: E402
+from .late import GATE_MAX_AGE, LateField, LateParams, TraceGate  # noqa: E402
 
-__all__ += ["Server"]
+__all__ += ["Server", "LateField", "LateParams", "TraceGate", "GATE_MAX_AGE"]

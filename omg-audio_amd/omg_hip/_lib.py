@@ -132,6 +132,27 @@ class RequestStats(C.Structure):
                 ("slot", C.c_uint32)]
 
 
+class TraceGateState(C.Structure):
+    _fields_ = [("last_src", C.c_float * 3), ("last_lis", C.c_float * 3),
+                ("last_energy", C.c_float * MAX_BANDS), ("age", C.c_uint32)]
+
+
+class LateUpdate(C.Structure):
+    _fields_ = [("slot", C.c_uint32), ("traced", C.c_uint32), ("cur_bins", C.c_void_p),
+                ("cur_dirs", C.c_void_p), ("src", C.c_float * 3), ("receiver", C.c_float * 3)]
+
+
+class LateTrace(C.Structure):
+    _fields_ = [("slot", C.c_uint32), ("params", TraceParams), ("src", C.c_float * 3),
+                ("receiver", C.c_float * 3)]
+
+
+class LateOut(C.Structure):
+    _fields_ = [("rt60", C.c_float * MAX_BANDS), ("level", C.c_float * MAX_BANDS),
+                ("dir", C.c_float * 3), ("aniso", C.c_float), ("version", C.c_uint32),
+                ("traced", C.c_uint32)]
+
+
 # (name, restype, argtypes) of every exported entry point
 _vp, _u32, _f32, _i = C.c_void_p, C.c_uint32, C.c_float, C.c_int
 SIGNATURES = [
@@ -177,6 +198,14 @@ SIGNATURES = [
     ("omg_server_replay", _i, [_vp, _vp, _vp, _u32, _vp, _vp, _vp]),
     ("omg_server_stats_get", _i, [_vp, _vp]),
     ("omg_server_destroy", None, [_vp]),
+    ("omg_trace_gate_init", None, [_vp, _u32]),
+    ("omg_trace_gate_step", _i, [_vp, _vp, _vp, _vp, _u32, _u32, C.POINTER(_i)]),
+    ("omg_late_field_create", _i, [_vp, _u32, _u32, _u32, _f32, C.POINTER(_vp)]),
+    ("omg_late_field_gate", _i, [_vp, _u32, _vp, _vp, _vp, _u32, C.POINTER(_i)]),
+    ("omg_late_field_update", _i, [_vp, _vp, _u32, _vp, _vp]),
+    ("omg_late_field_trace_world", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp]),
+    ("omg_late_field_read", _i, [_vp, _u32, _vp, _vp]),
+    ("omg_late_field_destroy", None, [_vp]),
     ("omg_estimate_reverb", _i, [_vp, _u32, _u32, _f32, _vp, _vp]),
     ("omg_late_direction", _i, [_vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp]),
     ("omg_dome_sample", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),

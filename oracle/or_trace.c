@@ -484,6 +484,74 @@ void or_estimate_reverb(const float *bins, uint32_t nbins, uint32_t nbands, floa
     }
 }
 
+/* ------------------------------------------------------------ Sim late path (sim.rs) */
+
+void or_gate_init(or_gate *g, uint32_t phase)
+{
+    /* last_src / last_lis = Vec3::new(f32::MAX, 0, 0), last_energy = 0 (sim.rs:47-52) */
+    for (int k = 0; k < 3; k++) {
+        g->last_src[k] = k == 0 ? FLT_MAX : 0.0f;
+        g->last_lis[k] = k == 0 ? FLT_MAX : 0.0f;
+    }
+    for (int b = 0; b < OR_MAXB; b++)
+        g->last_energy[b] = 0.0f;
+    g->age = phase;
+}
+
+int or_gate_step(or_gate *g, const float src[3], const float lis[3], const float *energy,
+                 uint32_t nbands, uint32_t max_age)
+{
+    g->age += 1;
+    int moved = 0;
+    for (uint32_t b = 0; b < nbands; b++) {
+        float e0 = g->last_energy[b], e1 = energy[b];
+        if (fabsf(e1 - e0) > 0.1f * fmaxf(fmaxf(e0, e1), 1e-6f))
+            moved = 1;
+    }
+    float ds = vlen(vsub(vload(src), vload(g->last_src)));
+    float dl = vlen(vsub(vload(lis), vload(g->last_lis)));
+    if (g->age >= max_age || ds > 0.25f || dl > 0.25f || moved) {
+        for (int k = 0; k < 3; k++) {
+            g->last_src[k] = src[k];
+            g->last_lis[k] = lis[k];
+        }
+        for (uint32_t b = 0; b < nbands; b++)
+            g->last_energy[b] = energy[b];
+        g->age = 0;
+        return 1;
+    }
+    return 0;
+}
+
+void or_late_slot_init(or_late_slot *s, float *bins, float *dirs)
+{
+    s->bins = bins;
+    s->dirs = dirs;
+    s->version = 0;
+    s->rp_dist = FLT_MAX;
+}
+
+void or_late_step(or_late_slot *s, uint32_t nbins, uint32_t nbands, float bin_dt, int traced,
+                  const float *cur_bins, const float *cur_dirs, const float src[3],
+                  const float receiver[3], float *rt60, float *level, float dir[3], float *aniso)
+{
+    v3 d = vsub(vload(src), vload(receiver));
+    if (traced) {
+        float alpha = s->version == 0 ? 1.0f : 0.3f;
+        or_ema(s->bins, s->dirs, cur_bins, cur_dirs, nbins, nbands, alpha);
+        s->rp_dist = vlen(d);
+    }
+    s->version += 1;
+    or_late_direction(s->bins, s->dirs, nbins, nbands, bin_dt, 0.08f, dir, aniso);
+    or_estimate_reverb(s->bins, nbins, nbands, bin_dt, rt60, level);
+    if (s->rp_dist < FLT_MAX) {
+        float d_now = vlen(d);
+        float k = fminf((s->rp_dist + 1.0f) / (d_now + 1.0f), 1.0f);
+        for (uint32_t b = 0; b < nbands; b++)
+            level[b] *= k;
+    }
+}
+
 /* ------------------------------------------------------------ ISM (ism.rs:15-84) */
 
 int or_image_source_taps(const float size[3], const or_material walls[6], uint32_t NB,

@@ -167,6 +167,31 @@ void or_late_direction(const float *bins, const float *dirs, uint32_t nbins, uin
 void or_estimate_reverb(const float *bins, uint32_t nbins, uint32_t nbands, float bin_dt,
                         float *rt60, float *level);
 
+/* ---------------------------------------------------------------- Sim late path */
+/* TraceGate (crates/omg-scene/src/sim.rs:38-76) */
+typedef struct {
+    float last_src[3], last_lis[3], last_energy[OR_MAXB];
+    uint32_t age;
+} or_gate;
+/* TraceGate::new (sim.rs:46-53) */
+void or_gate_init(or_gate *g, uint32_t phase);
+/* TraceGate::should_trace (sim.rs:55-75); returns 1 to trace */
+int or_gate_step(or_gate *g, const float src[3], const float lis[3], const float *energy,
+                 uint32_t nbands, uint32_t max_age);
+/* The late state of one Sim: echo_avg, version, rp_dist (sim.rs:79-112) */
+typedef struct {
+    float *bins, *dirs;
+    uint32_t version;
+    float rp_dist;
+} or_late_slot;
+void or_late_slot_init(or_late_slot *s, float *bins, float *dirs);
+/* Sim::reverb_world after the trace decision (sim.rs:374-397): EMA of the
+ * fresh echogram when traced, late_direction(0.08), estimate_reverb, the
+ * distance compensation of the level. */
+void or_late_step(or_late_slot *s, uint32_t nbins, uint32_t nbands, float bin_dt, int traced,
+                  const float *cur_bins, const float *cur_dirs, const float src[3],
+                  const float receiver[3], float *rt60, float *level, float dir[3], float *aniso);
+
 /* ---------------------------------------------------------------- ISM */
 /* ism::image_source_taps (ism.rs:15-84); taps as {key, delay, dir[3],
  * gains[nbands]} rows of (5 + nbands) floats.  Returns count. */

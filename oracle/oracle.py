@@ -116,6 +116,11 @@ def lib():
         L.or_late_direction.argtypes = [vp, vp, u32, u32, f32, f32, vp, vp]
         L.or_estimate_reverb.argtypes = [vp, u32, u32, f32, vp, vp]
         L.or_image_source_taps.argtypes = [vp, vp, u32, vp, vp, u32, vp, vp, C.c_int]
+        L.or_gate_init.argtypes = [vp, u32]
+        L.or_gate_step.restype = C.c_int
+        L.or_gate_step.argtypes = [vp, vp, vp, vp, u32, u32]
+        L.or_late_slot_init.argtypes = [vp, vp, vp]
+        L.or_late_step.argtypes = [vp, u32, u32, f32, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
         L.or_xorshift_next.restype = C.c_uint64
         L.or_xorshift_next.argtypes = [vp]
         L.or_xorshift_f32.restype = f32
@@ -391,6 +396,55 @@ def ema(bins, dirs, obins, odirs, alpha):
     n, nb = bins.shape
     lib().or_ema(_p(bins), _p(dirs), _p(_f32(obins)), _p(_f32(odirs)), n, nb,
                  float(np.float32(alpha)))
+
+
+class _Gate(C.Structure):
+    _fields_ = [("last_src", C.c_float * 3), ("last_lis", C.c_float * 3),
+                ("last_energy", C.c_float * MAXB), ("age", C.c_uint32)]
+
+
+class TraceGate:
+    """TraceGate (crates/omg-scene/src/sim.rs:38-76), or_gate_* in or_trace.c."""
+
+    def __init__(self, phase=0):
+        self.g = _Gate()
+        lib().or_gate_init(C.byref(self.g), phase)
+
+    def should_trace(self, src, lis, energy, max_age):
+        e = _f32(energy)
+        return bool(lib().or_gate_step(C.byref(self.g), _p(_f32(src)), _p(_f32(lis)), _p(e),
+                                       e.size, max_age))
+
+
+class _LateSlot(C.Structure):
+    _fields_ = [("bins", C.c_void_p), ("dirs", C.c_void_p), ("version", C.c_uint32),
+                ("rp_dist", C.c_float)]
+
+
+class LateSlot:
+    """One Sim's late state (echo_avg, version, rp_dist, sim.rs:79-112) and
+    the late path of Sim::reverb_world after the trace decision
+    (sim.rs:374-397), or_late_step in or_trace.c."""
+
+    def __init__(self, nbins, nbands, bin_dt):
+        self.bins = np.zeros((nbins, nbands), np.float32)
+        self.dirs = np.zeros((nbins, 3), np.float32)
+        self.nbins, self.nbands, self.bin_dt = nbins, nbands, float(np.float32(bin_dt))
+        self.s = _LateSlot()
+        lib().or_late_slot_init(C.byref(self.s), _p(self.bins), _p(self.dirs))
+
+    def step(self, traced, cur_bins, cur_dirs, src, receiver):
+        rt = np.zeros(self.nbands, np.float32)
+        lv = np.zeros(self.nbands, np.float32)
+        d = np.zeros(3, np.float32)
+        a = C.c_float()
+        cb = _f32(cur_bins) if traced else None
+        cd = _f32(cur_dirs) if traced else None
+        lib().or_late_step(C.byref(self.s), self.nbins, self.nbands, self.bin_dt, int(traced),
+                           _p(cb), _p(cd), _p(_f32(src)), _p(_f32(receiver)), _p(rt), _p(lv),
+                           _p(d), C.byref(a))
+        return dict(rt60=rt, level=lv, dir=d, aniso=np.float32(a.value),
+                    version=int(self.s.version))
 
 
 AIR_ABSORPTION_PER_M = [1.0e-4, 5.0e-4, 3.0e-3]   # material.rs:75
