@@ -19,6 +19,12 @@ import os
 import sys
 import time
 
+# The serving leg's slots and the tracer/dome pair run on their own HIP
+# streams besides the context's and torch's; HIP maps streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 by default) and streams that share a
+# queue serialize.  It must be set before HIP starts.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "omg-audio_amd"))
 
@@ -198,8 +204,9 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
               sources=r["sources"], listeners=r["listener"], dome=True) for r in reqs]
     arr0 = np.array([r["t"] for r in reqs])
     shape = dict(lanes=1, stack_lds=12, waves_per_simd=6, regen=1, width=4)
-    srv = H.Server(ctx, pool, hbm_budget=64 << 30, slots=2, max_batch=64, shape=shape,
-                   latency_rays=1 << 20)
+    slots, lat_rays = 2, 1 << 20
+    srv = H.Server(ctx, pool, hbm_budget=64 << 30, slots=slots, max_batch=64, shape=shape,
+                   latency_rays=lat_rays)
     srv.replay(R[:100], arr0[:100])
     rows = []
     for qps in qps_list:
@@ -223,7 +230,8 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
                         f"({min(mb.n_tris for mb, _, _ in specs)}-{max(mb.n_tris for mb, _, _ in specs)} tris), "
                         "Poisson arrivals, 1000 walkers at 1.4 m/s, 1-32 sources x 512-16384 rays "
                         "+ 512-ray dome per request, 64 GB budget",
-            "shape": shape, "slots": 2, "latency_rays": 1 << 20, "rates": rows,
+            "shape": shape, "slots": slots, "latency_rays": lat_rays,
+            "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"), "rates": rows,
             "max_qps_with_p99_under_10ms": max(held) if held else None,
             "batches": st["batches"], "mean_batch": st["mean_batch"],
             "peak_committed_bytes": st["peak_committed_bytes"]}

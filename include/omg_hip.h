@@ -417,8 +417,10 @@ typedef struct {
 
 typedef struct {
     uint64_t hbm_budget;        /* bytes: resident scenes + in-flight work arenas */
-    uint32_t slots;             /* batches in flight at once, 1..8; with 1-2 slots each slot
-                                   runs its dome fans on a second stream beside the tracer */
+    uint32_t slots;             /* batches in flight at once, 1..16, one stream each (with 1-2
+                                   slots a second stream runs the dome fans beside the tracer);
+                                   HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues
+                                   (4 by default): more slots than queues share them */
     uint32_t max_batch;         /* requests per batch, 1..4096 */
     uint32_t latency_rays;      /* batches of at most this many rays run a DPP quad per ray
                                    over 16-ray items (shorter per-ray chains); 0 = never */

@@ -56,6 +56,12 @@ using ReqP = std::shared_ptr<Req>;
 
 // rays per work item of a latency-mode batch: one pass of a wave of quads
 constexpr uint32_t kLatencyItemRays = 16;
+// First sizes of a slot's device arena and pinned staging.  Freeing device
+// or pinned memory waits for the whole device, so a slot that grew its
+// buffers a step at a time would stall every other slot's batch each time;
+// starting at these sizes (a C5 batch needs a few MB) makes growth rare.
+constexpr size_t kMinArenaBytes = 16u << 20;
+constexpr size_t kMinStagingBytes = 4u << 20;
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -77,7 +83,7 @@ int host_grow(void*& p, size_t& cap, size_t want)
     if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
-    size_t n = want + want / 4 + 4096;
+    size_t n = std::max(want + want / 4 + 4096, kMinStagingBytes);
     if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
         p = nullptr;
         return fail(OMG_ERR_DEVICE, "hipHostMalloc");
@@ -241,8 +247,8 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
 
     int rc = OMG_OK;
     if (total > S.arena.cap) {
-        // grow by doubling within the room the budget leaves this slot
-        uint64_t want = std::max<uint64_t>(total, 2 * (uint64_t)S.arena.cap);
+        // grow by doubling (from kMinArenaBytes) within the room the budget leaves this slot
+        uint64_t want = std::max<uint64_t>({total, 2 * (uint64_t)S.arena.cap, kMinArenaBytes});
         want = std::min<uint64_t>(want, std::max<uint64_t>(room, total));
         if (S.arena.ensure_exact((size_t)want) != hipSuccess)
             return fail(OMG_ERR_DEVICE, "work arena allocation");
@@ -478,7 +484,7 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
 {
     if (!c || !cfg || !out || (n_scenes && !scenes)) return fail(OMG_ERR_INVALID, "bad argument");
     *out = nullptr;
-    if (cfg->slots < 1 || cfg->slots > 8) return fail(OMG_ERR_INVALID, "slots must be 1..8");
+    if (cfg->slots < 1 || cfg->slots > 16) return fail(OMG_ERR_INVALID, "slots must be 1..16");
     if (cfg->max_batch < 1 || cfg->max_batch > 4096) return fail(OMG_ERR_INVALID, "max_batch must be 1..4096");
     BatchShape sh{cfg->shape.lanes, cfg->shape.stack_lds, cfg->shape.waves_per_simd, cfg->shape.regen,
                   cfg->shape.width};
