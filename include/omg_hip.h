@@ -563,6 +563,78 @@ int omg_late_field_trace_world(omg_late_field *lf, omg_scene *scene, const omg_l
 int omg_late_field_read(omg_late_field *lf, uint32_t slot, float *bins, float *dirs);
 void omg_late_field_destroy(omg_late_field *lf);
 
+/* ------------------------------------------------------------ flat wire formats
+ *
+ * The reference's web bridge queues traces as flat f32 jobs and takes
+ * echograms back as flat f32 buffers (crates/omg-web/src/lib.rs:151-161,
+ * :367-378, web/gpu.js:395-440); ParamBlock crosses thread and worker
+ * boundaries as a flat f32 layout (crates/omg-core/src/params.rs:62-143).
+ * These entry points take and give exactly those layouts, so a remote or
+ * asynchronous driver can hand its job queue to this library unchanged. */
+#define OMG_WIRE_BOX_JOB_F32S 57        /* GPU_JOB_F32S (lib.rs:157) */
+#define OMG_WIRE_BOX_JOB_VERSION 2      /* GPU_JOB_VERSION (lib.rs:158) */
+#define OMG_WIRE_WORLD_MAX_PANELS 64    /* WLATE_MAX_PANELS (lib.rs:373) */
+#define OMG_WIRE_WORLD_JOB_F32S (10 + OMG_WIRE_WORLD_MAX_PANELS * 16)  /* WLATE_JOB_F32S */
+#define OMG_WIRE_WORLD_JOB_VERSION 3    /* WLATE_VERSION (lib.rs:378) */
+#define OMG_WIRE_ECHO_BINS 300          /* MAX_TIME / BIN_DT (tracer.rs:12-14) */
+#define OMG_WIRE_ECHO_F32S (OMG_WIRE_ECHO_BINS * 3 + OMG_WIRE_ECHO_BINS * 3)  /* GPU_ECHO_F32S */
+#define OMG_WIRE_PARAM_HEADER 24        /* FLAT_HEADER (params.rs:74) */
+#define OMG_WIRE_PARAM_PER_TAP 9        /* FLAT_PER_TAP (params.rs:75) */
+
+/* One decoded world job (lib.rs:394-410 writes it; gpu.js:420-440 reads
+ * n_rays and seed as u32 and uses unit source energy). */
+typedef struct {
+    uint32_t id;
+    uint32_t n_rays;
+    uint32_t seed;
+    float src[3];
+    float lis[3];
+    uint32_t n_panels;       /* min(job[9], 64) */
+    omg_overlay panels[OMG_WIRE_WORLD_MAX_PANELS];
+} omg_wire_world_job;
+/* Decodes one 1034-f32 world job; OMG_ERR_VERSION unless version == 3. */
+int omg_wire_decode_world_job(const float *job, uint32_t version, omg_wire_world_job *out);
+
+/* Runs the flat box jobs (57 f32 each: id, n_rays, seed, room size, source,
+ * listener, band energy, 6 walls x (absorption x3, scattering,
+ * transmission x3), lib.rs:151-156, :168-185) on the analytic Shoebox with
+ * the reference tracer's constants (10 ms x 3 s, 64 bounces, floor 1e-7,
+ * receiver 0.5 m) and writes each job's echogram in the injection layout
+ * (300 x 3 bins then 300 x 3 dirs, lib.rs:159-160) to echo_out[k * 1800],
+ * its sim id to ids_out[k].  n_f32 must be a multiple of 57. */
+int omg_wire_trace_box_jobs(omg_ctx *ctx, const float *jobs, uint32_t n_f32, uint32_t version,
+                            float *echo_out, uint32_t *ids_out, uint32_t *n_jobs_out);
+/* The same for flat world jobs (1034 f32 each) on a 3-band mesh scene,
+ * panels as overlays. */
+int omg_wire_trace_world_jobs(omg_scene *scene, const float *jobs, uint32_t n_f32,
+                              uint32_t version, float *echo_out, uint32_t *ids_out,
+                              uint32_t *n_jobs_out);
+
+/* ParamBlock (params.rs:62-70) with its taps (params.rs:8-16) and the
+ * optional RemoteReverb (params.rs:46-55). */
+typedef struct {
+    uint32_t key;
+    float delay_s;
+    float dir[3];
+    float gains[3];
+} omg_tap;
+typedef struct {
+    uint64_t version;
+    float rt60[3];
+    float level[3];
+    uint32_t has_remote;
+    float remote_rt60[3];
+    float remote_send[3];
+    float remote_sh[9];
+} omg_param_block;
+/* ParamBlock::write_flat (params.rs:82-107): 24 + 9 * n_taps f32. */
+int omg_param_block_write_flat(const omg_param_block *pb, const omg_tap *taps, uint32_t n_taps,
+                               float *out, uint32_t cap, uint32_t *n_out);
+/* ParamBlock::read_flat (params.rs:109-142); *n_taps is the flat block's
+ * tap count, taps receives at most cap of them. */
+int omg_param_block_read_flat(const float *data, uint32_t n_f32, omg_param_block *pb,
+                              omg_tap *taps, uint32_t cap, uint32_t *n_taps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,8 @@ The host-side mirror of the reference interface over the C ABI
 - ``estimate_reverb`` / ``late_direction`` (crates/omg-core/src/tracer.rs:205-257, :62-77)
 - ``LateField.reverb_world`` / ``TraceGate`` = the late path of ``Sim::reverb_world``
   (crates/omg-scene/src/sim.rs:38-76, :349-397), averages resident on the device
+- ``wire``: the web bridge's flat job / echogram layouts and ParamBlock::write_flat
+  (crates/omg-web/src/lib.rs:151-161, :367-378; crates/omg-core/src/params.rs:62-143)
 
 Like the reference backends, a refused call raises ``OmgError`` (the
 reference returns ``false``/``None``); construction raises when no GPU is
@@ -392,5 +394,6 @@ from .batch import Batch, QuerySpec  # noqa: E402
 __all__ += ["Batch", "QuerySpec"]
 from .serve import Server  # noqa: E402
 from .late import GATE_MAX_AGE, LateField, LateParams, TraceGate  # noqa: E402
+from . import wire  # noqa: E402,F401
 
-__all__ += ["Server", "LateField", "LateParams", "TraceGate", "GATE_MAX_AGE"]
+__all__ += ["Server", "LateField", "LateParams", "TraceGate", "GATE_MAX_AGE", "wire"]

@@ -153,6 +153,34 @@ class LateOut(C.Structure):
                 ("traced", C.c_uint32)]
 
 
+WIRE_BOX_JOB_F32S = 57
+WIRE_BOX_JOB_VERSION = 2
+WIRE_WORLD_MAX_PANELS = 64
+WIRE_WORLD_JOB_F32S = 10 + WIRE_WORLD_MAX_PANELS * 16
+WIRE_WORLD_JOB_VERSION = 3
+WIRE_ECHO_BINS = 300
+WIRE_ECHO_F32S = WIRE_ECHO_BINS * 6
+WIRE_PARAM_HEADER = 24
+WIRE_PARAM_PER_TAP = 9
+
+
+class WireWorldJob(C.Structure):
+    _fields_ = [("id", C.c_uint32), ("n_rays", C.c_uint32), ("seed", C.c_uint32),
+                ("src", C.c_float * 3), ("lis", C.c_float * 3), ("n_panels", C.c_uint32),
+                ("panels", Overlay * WIRE_WORLD_MAX_PANELS)]
+
+
+class Tap(C.Structure):
+    _fields_ = [("key", C.c_uint32), ("delay_s", C.c_float), ("dir", C.c_float * 3),
+                ("gains", C.c_float * 3)]
+
+
+class ParamBlock(C.Structure):
+    _fields_ = [("version", C.c_uint64), ("rt60", C.c_float * 3), ("level", C.c_float * 3),
+                ("has_remote", C.c_uint32), ("remote_rt60", C.c_float * 3),
+                ("remote_send", C.c_float * 3), ("remote_sh", C.c_float * 9)]
+
+
 # (name, restype, argtypes) of every exported entry point
 _vp, _u32, _f32, _i = C.c_void_p, C.c_uint32, C.c_float, C.c_int
 SIGNATURES = [
@@ -206,6 +234,11 @@ SIGNATURES = [
     ("omg_late_field_trace_world", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp]),
     ("omg_late_field_read", _i, [_vp, _u32, _vp, _vp]),
     ("omg_late_field_destroy", None, [_vp]),
+    ("omg_wire_decode_world_job", _i, [_vp, _u32, _vp]),
+    ("omg_wire_trace_box_jobs", _i, [_vp, _vp, _u32, _u32, _vp, _vp, C.POINTER(_u32)]),
+    ("omg_wire_trace_world_jobs", _i, [_vp, _vp, _u32, _u32, _vp, _vp, C.POINTER(_u32)]),
+    ("omg_param_block_write_flat", _i, [_vp, _vp, _u32, _vp, _u32, C.POINTER(_u32)]),
+    ("omg_param_block_read_flat", _i, [_vp, _u32, _vp, _vp, _u32, C.POINTER(_u32)]),
     ("omg_estimate_reverb", _i, [_vp, _u32, _u32, _f32, _vp, _vp]),
     ("omg_late_direction", _i, [_vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp]),
     ("omg_dome_sample", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
