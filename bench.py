@@ -19,11 +19,12 @@ import os
 import sys
 import time
 
-# The serving leg's slots and the tracer/dome pair run on their own HIP
-# streams besides the context's and torch's; HIP maps streams onto
-# GPU_MAX_HW_QUEUES hardware queues (4 by default) and streams that share a
-# queue serialize.  It must be set before HIP starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# The serving leg keeps 8 batches in flight, each on a tracer stream and a
+# dome stream, besides the context's and torch's streams; HIP maps streams
+# onto GPU_MAX_HW_QUEUES hardware queues (4 by default) and streams that
+# share a queue serialize.  It must be set before HIP starts (and the
+# environment of the GPU hosts sets it to 4, so it is overridden here).
+os.environ["GPU_MAX_HW_QUEUES"] = "24"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "omg-audio_amd"))
@@ -204,10 +205,13 @@ def serving(ctx, qps_list=(100, 250, 500, 1000, 2000), seconds=1.0):
               sources=r["sources"], listeners=r["listener"], dome=True) for r in reqs]
     arr0 = np.array([r["t"] for r in reqs])
     shape = dict(lanes=1, stack_lds=12, waves_per_simd=6, regen=1, width=4)
-    slots, lat_rays = 2, 1 << 20
+    slots, lat_rays = 8, 1 << 18
     srv = H.Server(ctx, pool, hbm_budget=64 << 30, slots=slots, max_batch=64, shape=shape,
                    latency_rays=lat_rays)
+    # Warm-up: 1000 arrivals at the top rate grow every slot's arena
+    # and staging to their working size, so no measured rate pays for it.
     srv.replay(R[:100], arr0[:100])
+    srv.replay(R[:1000], arr0[:1000])
     rows = []
     for qps in qps_list:
         n = min(len(R), int(qps * seconds))

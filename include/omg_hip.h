@@ -417,13 +417,14 @@ typedef struct {
 
 typedef struct {
     uint64_t hbm_budget;        /* bytes: resident scenes + in-flight work arenas */
-    uint32_t slots;             /* batches in flight at once, 1..16, one stream each (with 1-2
-                                   slots a second stream runs the dome fans beside the tracer);
-                                   HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues
-                                   (4 by default): more slots than queues share them */
+    uint32_t slots;             /* batches in flight at once, 1..16.  Each slot runs its tracer
+                                   and its dome fans on two streams while 2 * slots + 2 fit
+                                   GPU_MAX_HW_QUEUES (HIP's hardware queues per process, 4 by
+                                   default; always with 1-2 slots), else on one stream */
     uint32_t max_batch;         /* requests per batch, 1..4096 */
     uint32_t latency_rays;      /* batches of at most this many rays run a DPP quad per ray
-                                   over 16-ray items (shorter per-ray chains); 0 = never */
+                                   with regeneration over 16-ray items (shorter per-ray chains,
+                                   lanes refilled as rays end); 0 = never */
     omg_batch_shape shape;      /* persistent launch shape of the other batches */
     omg_dome_params dome;       /* dome fan of requests with dome = 1 */
 } omg_server_config;

@@ -12,6 +12,7 @@
 // scenes plus the slots' grow-only work arenas; an arena only grows while
 // the total stays within the budget.
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -173,15 +174,16 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
     for (uint32_t g = 0; g < ng; g++)
         for (uint32_t i : gmem[g]) g_eyes[g] += batch[i]->n_lis;
 
-    // small batches run in latency mode: a DPP quad per ray over 16-ray items
-    // (the whole traversal stack in LDS where the trees allow it)
+    // small batches run in latency mode: a DPP quad per ray (the whole
+    // traversal stack in LDS where the trees allow it) with regeneration
+    // over 16-ray items; the grid is sized to the batch
     uint64_t rays = 0;
     for (uint32_t i = 0; i < n; i++) rays += (uint64_t)batch[i]->n_src * batch[i]->p.n_rays;
     const bool lat = cfg.latency_rays && rays <= cfg.latency_rays;
     const uint32_t item_rays = lat ? kLatencyItemRays : kBatchItemRays;
     const BatchShape sh = !lat                        ? shape
-                          : deep && shape.width == 4 ? BatchShape{4, kDeepStack, 8, 0, 4}
-                                                     : BatchShape{4, 16, 8, 0, shape.width};
+                          : deep && shape.width == 4 ? BatchShape{4, kDeepStack, 8, 1, 4}
+                                                     : BatchShape{4, 16, 8, 1, shape.width};
 
     // inputs
     std::vector<size_t> o_mats(n), o_src(n), o_lis(n), g_eye_off(ng);
@@ -307,7 +309,7 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         OMG_HIP_CHECK(hipStreamWaitEvent(S.side, S.fork, 0));
     }
     // a grid no larger than the batch needs leaves the rest of the chip to the other slots
-    uint64_t lanes_needed = lat ? items * 64 : rays * sh.lanes;
+    uint64_t lanes_needed = rays * sh.lanes;
     uint32_t max_blocks = (uint32_t)std::min<uint64_t>((lanes_needed + 255) / 256, 1u << 20);
     if (S.side) OMG_HIP_CHECK(launch_trace_batch(B, nbm, sh, ctx->n_cu, false, st, max_blocks));
     for (uint32_t g = 0; g < ng; g++) {
@@ -516,9 +518,15 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
     s->scenes_bytes = sb;
     s->slots.resize(cfg->slots);
     int rc = OMG_OK;
-    // two streams per slot (tracer, dome) while they fit the device's 4
-    // hardware queues; with more slots each slot runs on one stream
-    const bool side = cfg->slots <= 2;
+    // two streams per slot (tracer, dome) while they fit the process's
+    // hardware queues (GPU_MAX_HW_QUEUES, 4 by default, two kept for the
+    // context's and the caller's streams); otherwise one stream per slot
+    uint32_t hw_queues = 4;
+    if (const char* q = std::getenv("GPU_MAX_HW_QUEUES")) {
+        long v = std::strtol(q, nullptr, 10);
+        if (v > 0 && v < 1024) hw_queues = (uint32_t)v;
+    }
+    const bool side = cfg->slots <= 2 || 2 * cfg->slots + 2 <= hw_queues;
     for (Slot& S : s->slots) {
         if (hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking) != hipSuccess ||
             (side && hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking) != hipSuccess) ||
