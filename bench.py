@@ -283,6 +283,52 @@ def late_field(ctx, dev, n_slots=4096, reps=5):
             "rt60_slot0": [float(x) for x in out[0].rt60[:3]]}
 
 
+def _same_tree(a, b):
+    """Device build vs host build: primitives bit for bit, node references
+    exact, node boxes equal as values."""
+    import numpy as np
+    (pa, na, n4a, sa), (pb, nb, n4b, sb) = a, b
+    if sa != sb or not np.array_equal(pa.view(np.uint32), pb.view(np.uint32)):
+        return False
+    for x, y in ((na, nb), (n4a, n4b)):
+        if x.shape != y.shape:
+            return False
+        refs = [3, 7, 11, 15] if x.shape[1] == 16 else [c for c in range(32) if c % 8 in (3, 7)]
+        boxes = [c for c in range(x.shape[1]) if c not in refs]
+        if not (np.array_equal(x.view(np.uint32)[:, refs], y.view(np.uint32)[:, refs]) and
+                np.array_equal(x[:, boxes], y[:, boxes])):
+            return False
+    return True
+
+
+def scene_build_leg(ctx, host_scene, mesh, reps=3):
+    """The same scene built on the device (omg_scene_create_device: the
+    tessellation, the SAH tree and the 4-wide collapse as kernels, inputs
+    from host memory), timed on the C call after one warm-up build, checked
+    against the host build; then refits of the unchanged vertices."""
+    import omg_hip as H
+    ms = []
+    dev = None
+    for k in range(reps + 1):
+        if dev is not None:
+            dev.close()
+        t0 = time.perf_counter()
+        dev = H.Scene.from_mesh(ctx, nbands=6, device=True, **mesh)
+        if k:
+            ms.append((time.perf_counter() - t0) * 1e3)
+    same = _same_tree(host_scene.export(), dev.export())
+    refit = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dev.refit(mesh["positions"])
+        refit.append((time.perf_counter() - t0) * 1e3)
+    dev.close()
+    ms.sort()
+    refit.sort()
+    return {"ms": ms[len(ms) // 2], "ms_all": ms, "refit_ms": refit[len(refit) // 2],
+            "identical_to_host_build": same}
+
+
 def stress(ctx, dev, launches=2, fixed_shape=None):
     """Large-scene stress: the ~2M-triangle town (400 buildings with
     storey slabs, partitions and apertures), 64 sources x 1M rays x 6
@@ -297,10 +343,18 @@ def stress(ctx, dev, launches=2, fixed_shape=None):
     t0 = time.perf_counter()
     mb, src, lis = scenes.c4_workload()
     gen_s = time.perf_counter() - t0
+    soup = mb.soup()
     t0 = time.perf_counter()
-    scene = H.Scene.from_builder(ctx, mb, nbands=6)
+    pos, idx = H.dedup_soup(soup["tri_verts"])
+    dedup_s = time.perf_counter() - t0
+    mesh = dict(positions=pos, indices=idx, tri_material=soup["tri_material"],
+                materials=soup["materials"], tri_surface=soup["tri_surface"],
+                tri_max_edge=soup["tri_max_edge"])
+    t0 = time.perf_counter()
+    scene = H.Scene.from_mesh(ctx, nbands=6, **mesh)
     build_s = time.perf_counter() - t0
     info = scene.info()
+    build = scene_build_leg(ctx, scene, mesh)
     n_rays = 1 << 20
     params = H.trace_params(n_rays, 6, max_bounces=128, floor_rel=1e-7, seed=0xC4, bin_dt=0.001,
                             max_time=2.0)
@@ -349,7 +403,10 @@ def stress(ctx, dev, launches=2, fixed_shape=None):
                     f"{n_src} sources x {n_rays} rays x 6 bands at {n_lis} listeners, "
                     "128 bounces, floor 1e-7",
         "tris": info["n_tris"], "prims": info["n_prims"], "bvh_nodes": info["n_nodes"],
-        "scene_device_bytes": info["device_bytes"], "generate_s": gen_s, "build_s": build_s,
+        "scene_device_bytes": info["device_bytes"], "generate_s": gen_s, "dedup_s": dedup_s,
+        "build_s": build_s,
+        "device_build": dict(build, host_build_s=build_s,
+                             speedup_vs_host=build_s / (build["ms"] * 1e-3)),
         "bounce_rays": br, "aabb_tests": tc["aabb_tests"], "tri_tests": tc["tri_tests"],
         "plain": {"ms": pm * 1e3, "bounce_rays_per_s": br / pm,
                   "roofline_frac": alg / pm / 1e9 / HBM_PEAK_GBS},

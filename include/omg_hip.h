@@ -166,6 +166,47 @@ int omg_bvh4_build_host(const float *positions, uint32_t nv, const uint32_t *ind
                         const float *tri_max_edge, float *nodes4_out, uint32_t node_cap,
                         uint32_t *n_nodes4, uint32_t *root4, uint32_t *stack4, uint32_t *usable);
 
+/* The same scene built on the device: Mesh::with_surfaces_edges +
+ * build_bvh (mesh.rs:198-405) with the tessellation, the 16-bin SAH tree,
+ * its breadth-first flatten and the 4-wide collapse computed by kernels,
+ * level by level.  Same arguments and errors as omg_scene_create (a soup
+ * goes through omg_dedup_soup first), and the same scene: primitives,
+ * paired nodes and 4-wide nodes equal the host build's (box coordinates
+ * equal as values; a zero may carry the other sign).  In addition it
+ * refuses non-finite vertex coordinates (OMG_ERR_INVALID) and patches
+ * more than 60 splits deep (OMG_ERR_CAP), and keeps its inputs on the
+ * device for omg_scene_refit. */
+int omg_scene_create_device(omg_ctx *ctx, const float *positions, uint32_t nv,
+                            const uint32_t *indices, uint32_t nt, const uint16_t *tri_material,
+                            const uint16_t *tri_surface, const float *tri_max_edge,
+                            const omg_material *materials, uint32_t nm, uint32_t nbands,
+                            omg_scene **out);
+
+typedef struct {
+    uint32_t n_prims, n_nodes, n_nodes4;
+    uint32_t root_ref, root4, stack4, has4, max_depth, empty, pad;
+    float root_min[3], root_max[3];
+} omg_scene_tree;
+
+/* The resident tree of any scene, in the row layouts of omg_bvh_build_host
+ * and omg_bvh4_build_host (n_nodes4 is 0 when the scene walks width 2
+ * only); each output may be NULL. */
+int omg_scene_export(omg_scene *scene, float *prims_out, uint32_t prim_cap, float *nodes_out,
+                     uint32_t node_cap, float *nodes4_out, uint32_t node4_cap,
+                     omg_scene_tree *tree);
+
+/* Refit for moving geometry (no reference counterpart: the reference
+ * rebuilds, and keeps door leaves as box overlays to avoid it,
+ * scene.rs:84-89).  New positions [nv][3] for a scene built by
+ * omg_scene_create_device, same nv and triangles: the patches are
+ * re-derived from the moved vertices and every box of both trees is
+ * recomputed bottom-up; the tree topology is kept.  Refused
+ * (OMG_ERR_INVALID, scene unchanged) when a triangle's patch count would
+ * change -- build the scene again then.  Nearest hits are the same as
+ * with a fresh build of the moved mesh except among hits at exactly equal
+ * distance. */
+int omg_scene_refit(omg_scene *scene, const float *positions, uint32_t nv);
+
 void omg_scene_destroy(omg_scene *scene);
 
 /* ------------------------------------------------------------ late-field tracer */

@@ -20,6 +20,7 @@ static_assert(sizeof(omg_mesh_record) == sizeof(MeshRecordDev), "record layout")
 extern "C" int omg_surface_table(omg_scene* s, omg_surface* out, uint32_t cap, uint32_t* n_out)
 {
     if (!s || !n_out || (cap && !out)) return fail(OMG_ERR_INVALID, "bad argument");
+    if (int rc = ensure_host_mesh(s)) return rc;
     std::vector<omg_surface> tab;
     for (uint32_t t = 0; t < s->n_tris; t++) {
         uint32_t sid = s->tri_surface[t];

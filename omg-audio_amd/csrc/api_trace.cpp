@@ -291,6 +291,9 @@ extern "C" void omg_scene_destroy(omg_scene* s)
     s->mats_raw.release();
     s->tri_surf.release();
     s->nodes4.release();
+    for (DevBuf* b : {&s->b_pos, &s->b_idx, &s->b_mat, &s->b_surf, &s->b_maxe, &s->b_cnt,
+                      &s->b_off, &s->leaf_of, &s->src4})
+        b->release();
     omg_ctx* c = s->ctx;
     delete s;
     ctx_release(c);

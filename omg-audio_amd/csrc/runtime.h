@@ -111,6 +111,13 @@ struct omg_scene {
     uint32_t n_nodes4 = 0, root4 = 0, stack4 = 0;
     bool has4 = false;
     uint64_t device_bytes = 0;
+    // scenes built on the device keep what a refit needs: the inputs, the
+    // per-triangle patch counts and offsets, each patch's leaf position,
+    // the first node of every tree level and each 4-wide child's source
+    bool device_built = false;
+    uint32_t nv = 0;
+    omg::DevBuf b_pos, b_idx, b_mat, b_surf, b_maxe, b_cnt, b_off, leaf_of, src4;
+    std::vector<uint32_t> level_base;
 };
 
 namespace omg {
@@ -130,4 +137,7 @@ void ctx_release(omg_ctx* c);
 void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins);
 // waits for in-flight *_async work before shared staging is overwritten
 int quiesce(omg_ctx* c);
+// host copies of the mesh and its triangle normals (scenes built on the
+// device download and compute them on first use)
+int ensure_host_mesh(omg_scene* s);
 }  // namespace omg

@@ -25,7 +25,7 @@ import numpy as np
 
 from ._lib import (DOME_BINS, LAYOUT_VERSION, MAX_BANDS, MAX_LISTENERS, MAX_OVERLAYS, Blocker,
                    Counters, DomePanel, DomeParams, DomeRay, MeshRecord, OmgError, Overlay,
-                   RayRecord, SceneInfo, Surface, TraceParams, check, lib)
+                   RayRecord, SceneInfo, SceneTree, Surface, TraceParams, check, lib)
 from . import _lib
 
 __all__ = ["Context", "Scene", "MaterialDef", "trace_params", "trace_world", "trace_box",
@@ -35,6 +35,17 @@ __all__ = ["Context", "Scene", "MaterialDef", "trace_params", "trace_world", "tr
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def dedup_soup(tri_verts):
+    """MeshBuilder's 0.1 mm vertex dedup (omg_dedup_soup): positions [nv][3], indices [nt][3]."""
+    tv = np.ascontiguousarray(np.asarray(tri_verts, np.float32).reshape(-1, 9))
+    nt = tv.shape[0]
+    pos = np.zeros((max(nt, 1) * 3, 3), np.float32)
+    idx = np.zeros((nt, 3), np.uint32)
+    nv = C.c_uint32()
+    check(lib().omg_dedup_soup(_p(tv), nt, _p(pos), _p(idx), C.byref(nv)))
+    return pos[:nv.value].copy(), idx
 
 
 def _f32(a, shape=None):
@@ -123,7 +134,9 @@ class Scene:
 
     @classmethod
     def from_mesh(cls, ctx, positions, indices, tri_material, materials, nbands=3,
-                  tri_surface=None, tri_max_edge=None):
+                  tri_surface=None, tri_max_edge=None, device=False):
+        """device=True builds the tessellation and both trees on the GPU
+        (omg_scene_create_device): the same scene, and refittable."""
         pos = _f32(positions).reshape(-1, 3)
         idx = np.ascontiguousarray(np.asarray(indices, np.uint32).reshape(-1, 3))
         nt = idx.shape[0]
@@ -132,16 +145,20 @@ class Scene:
         te = None if tri_max_edge is None else _f32(tri_max_edge)
         mats = _materials(materials)
         h = C.c_void_p()
-        check(lib().omg_scene_create(ctx.h, _p(pos), pos.shape[0], _p(idx), nt, _p(tm), _p(ts),
-                                     _p(te), C.cast(mats, C.c_void_p), len(materials), nbands,
-                                     C.byref(h)))
+        fn = lib().omg_scene_create_device if device else lib().omg_scene_create
+        check(fn(ctx.h, _p(pos), pos.shape[0], _p(idx), nt, _p(tm), _p(ts), _p(te),
+                 C.cast(mats, C.c_void_p), len(materials), nbands, C.byref(h)))
         return cls(ctx, h, nbands, len(materials))
 
     @classmethod
     def from_soup(cls, ctx, tri_verts, tri_material, materials, nbands=3, tri_surface=None,
-                  tri_max_edge=None):
+                  tri_max_edge=None, device=False):
         tv = _f32(tri_verts).reshape(-1, 9)
         nt = tv.shape[0]
+        if device:
+            pos, idx = dedup_soup(tv)
+            return cls.from_mesh(ctx, pos, idx, tri_material, materials, nbands, tri_surface,
+                                 tri_max_edge, device=True)
         tm = np.ascontiguousarray(np.asarray(tri_material, np.uint16))
         ts = None if tri_surface is None else np.ascontiguousarray(np.asarray(tri_surface, np.uint16))
         te = None if tri_max_edge is None else _f32(tri_max_edge)
@@ -153,10 +170,30 @@ class Scene:
         return cls(ctx, h, nbands, len(materials))
 
     @classmethod
-    def from_builder(cls, ctx, mb, nbands=3):
+    def from_builder(cls, ctx, mb, nbands=3, device=False):
         soup = mb.soup()
         return cls.from_soup(ctx, soup["tri_verts"], soup["tri_material"], soup["materials"],
-                             nbands, soup["tri_surface"], soup["tri_max_edge"])
+                             nbands, soup["tri_surface"], soup["tri_max_edge"], device=device)
+
+    def export(self):
+        """The resident trees: prims [n][12] and nodes [n][16] as f32 rows
+        (omg_bvh_build_host's layout), nodes4 [n][32], and the tree summary."""
+        t = SceneTree()
+        check(lib().omg_scene_export(self.h, None, 0, None, 0, None, 0, C.byref(t)))
+        prims = np.zeros((t.n_prims, 12), np.float32)
+        nodes = np.zeros((t.n_nodes, 16), np.float32)
+        nodes4 = np.zeros((t.n_nodes4, 32), np.float32)
+        check(lib().omg_scene_export(self.h, _p(prims), t.n_prims, _p(nodes), t.n_nodes,
+                                     _p(nodes4), t.n_nodes4, C.byref(t)))
+        summary = {k: getattr(t, k) for k, _ in SceneTree._fields_ if k not in ("pad",)}
+        summary["root_min"] = list(t.root_min)
+        summary["root_max"] = list(t.root_max)
+        return prims, nodes, nodes4, summary
+
+    def refit(self, positions):
+        """Moves the vertices of a scene built with device=True (omg_scene_refit)."""
+        pos = _f32(positions).reshape(-1, 3)
+        check(lib().omg_scene_refit(self.h, _p(pos), pos.shape[0]))
 
     def info(self):
         i = SceneInfo()

@@ -60,6 +60,13 @@ class SceneInfo(C.Structure):
                 ("max_depth", C.c_uint32), ("device_bytes", C.c_uint64)]
 
 
+class SceneTree(C.Structure):
+    _fields_ = [("n_prims", C.c_uint32), ("n_nodes", C.c_uint32), ("n_nodes4", C.c_uint32),
+                ("root_ref", C.c_uint32), ("root4", C.c_uint32), ("stack4", C.c_uint32),
+                ("has4", C.c_uint32), ("max_depth", C.c_uint32), ("empty", C.c_uint32),
+                ("pad", C.c_uint32), ("root_min", C.c_float * 3), ("root_max", C.c_float * 3)]
+
+
 class DomePanel(C.Structure):
     _fields_ = [("axis", C.c_uint32), ("plane", C.c_float), ("lat0", C.c_float),
                 ("lat1", C.c_float), ("z0", C.c_float), ("z1", C.c_float),
@@ -201,6 +208,10 @@ SIGNATURES = [
     ("omg_bvh4_build_host", _i, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32,
                                  C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32),
                                  C.POINTER(_u32)]),
+    ("omg_scene_create_device", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32,
+                                     C.POINTER(_vp)]),
+    ("omg_scene_export", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp]),
+    ("omg_scene_refit", _i, [_vp, _vp, _u32]),
     ("omg_scene_destroy", None, [_vp]),
     ("omg_trace_world", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("omg_trace_world_hits", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp,

@@ -218,3 +218,54 @@ def test_bvh4_collapse_covers_the_binary_tree(which):
     assert sorted(seen) == sorted(leaf_box)
     assert len(need) == n4.shape[0]
     assert usable == (1 if stack4 + 1 <= 64 else 0)
+
+
+def _two_pointer(left):
+    """The host partition (mesh.rs:384-395): scan from the front, swap each
+    right-side element with the back."""
+    order = list(range(len(left)))
+    i, j = 0, len(order)
+    while i < j:
+        if left[order[i]]:
+            i += 1
+        else:
+            j -= 1
+            order[i], order[j] = order[j], order[i]
+    return order
+
+
+def _closed_form(left):
+    """Destination of every element as the device build computes it from
+    prefix counts (build_kernels.hip, k_ptable / k_permute)."""
+    n = len(left)
+    L = np.asarray(left, bool)
+    nl = int(L.sum())
+    lb = np.concatenate([[0], np.cumsum(L)[:-1]]).astype(int)
+    c = nl - int(L[:nl].sum())
+    a_r = np.zeros(n, int)
+    b_l = np.zeros(n, int)
+    for p in range(n):
+        if not L[p] and p < nl:
+            a_r[(p - lb[p])] = p
+        if L[p] and p >= nl:
+            b_l[nl - 1 - lb[p]] = p
+    out = [None] * n
+    for p in range(n):
+        if L[p]:
+            d = p if p < nl else a_r[nl - 1 - lb[p]]
+        else:
+            rb = p - lb[p]
+            bprev = n if rb == 0 else b_l[rb - 1]
+            bc = n if c == 0 else b_l[c - 1]
+            d = bprev - 1 if rb < c else (bc - 1 if rb == c and p < bc else p - 1)
+        assert out[d] is None
+        out[d] = p
+    return out
+
+
+def test_partition_closed_form_matches_two_pointer():
+    rng = np.random.default_rng(5)
+    for _ in range(3000):
+        n = int(rng.integers(1, 48))
+        left = list(rng.random(n) < rng.random())
+        assert _closed_form(left) == _two_pointer(left), left
