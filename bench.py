@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-stress", action="store_true", help="skip the 2M-triangle town")
     ap.add_argument("--no-late", action="store_true", help="skip the late-field state workload")
+    ap.add_argument("--build-only", action="store_true",
+                    help="only the town's scene build, on the host and on the device")
     ap.add_argument("--batch-shape", default="",
                     help="lanes,stack_lds,waves_per_simd,regen,width for the batched launches "
                          "instead of the autotuner's choice (reproducible profiles)")
@@ -329,6 +331,28 @@ def scene_build_leg(ctx, host_scene, mesh, reps=3):
             "identical_to_host_build": same}
 
 
+def build_only(ctx):
+    """The town's scene (about 2.07M triangles) built on the host and on
+    the device; see scene_build_leg."""
+    import omg_hip as H
+    from omg_hip import scenes
+    mb, _, _ = scenes.c4_workload()
+    soup = mb.soup()
+    pos, idx = H.dedup_soup(soup["tri_verts"])
+    mesh = dict(positions=pos, indices=idx, tri_material=soup["tri_material"],
+                materials=soup["materials"], tri_surface=soup["tri_surface"],
+                tri_max_edge=soup["tri_max_edge"])
+    t0 = time.perf_counter()
+    host = H.Scene.from_mesh(ctx, nbands=6, **mesh)
+    host_s = time.perf_counter() - t0
+    info = host.info()
+    leg = scene_build_leg(ctx, host, mesh)
+    host.close()
+    return dict(leg, workload="town scene build", tris=info["n_tris"], prims=info["n_prims"],
+                bvh_nodes=info["n_nodes"], host_build_s=host_s,
+                speedup_vs_host=host_s / (leg["ms"] * 1e-3))
+
+
 def stress(ctx, dev, launches=2, fixed_shape=None):
     """Large-scene stress: the ~2M-triangle town (400 buildings with
     storey slabs, partitions and apertures), 64 sources x 1M rays x 6
@@ -419,8 +443,8 @@ def stress(ctx, dev, launches=2, fixed_shape=None):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the path shards by source/query with no communication; see DESIGN.md)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by source and by query with no communication between GPUs)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
@@ -430,6 +454,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     ctx = H.Context(0)
+    if args.build_only:
+        print(json.dumps(build_only(ctx)), flush=True)
+        return
     mb, src, lis = scenes.c2_workload()
     scene = H.Scene.from_builder(ctx, mb)
     info = scene.info()
