@@ -270,6 +270,35 @@ int or_mesh_record(const or_mesh *m, const or_surface *table, int n_surf, const 
 int or_mesh_chains(const or_mesh *m, const float lis[3], uint32_t n_rays, uint32_t rot,
                    int use_project_trig, uint16_t *chains, uint8_t *orders, int cap);
 
+/* ---------------------------------------------------------------- knife-edge paths */
+/* DiffractionEdge (mesh.rs:626-634) */
+typedef struct {
+    float a[3], b[3];
+    float importance;
+} or_edge;
+/* PathBudget (paths.rs:22-45) */
+typedef struct {
+    uint32_t edge_candidates, pair_edges, max_paths, taps;
+} or_path_budget;
+#define OR_PATH_MAX_POINTS 25
+/* FoundPath (paths.rs:48-60) with at most 25 points (the silhouette's hull) */
+typedef struct {
+    uint32_t key, n_points;
+    float length;
+    float gains[3];
+    float points[OR_PATH_MAX_POINTS][3];
+} or_path;
+/* extract_edges + budget_edges (mesh.rs:639-682); edges of equal
+ * importance in ascending vertex-pair order */
+int or_paths_edges(const or_mesh *m, float min_dihedral_deg, uint32_t budget, or_edge *out, int cap);
+void or_paths_transmission(const or_mesh *m, const float a[3], const float b[3], float g[3]);
+float or_knife_edge_amp(float detour_m, float freq_hz);                  /* diffraction.rs:27-42 */
+void or_knife_edge_bands(float detour_m, float out[3]);                  /* diffraction.rs:45-47 */
+void or_bend_point(const float a[3], const float b[3], const float src[3], const float lis[3],
+                   float p[3]);                                          /* paths.rs:411-436 */
+int or_paths_find(const or_mesh *m, const or_edge *edges, int ne, const float src[3],
+                  const float lis[3], const or_path_budget *b, or_path *out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,15 @@ def lib():
         L.or_mesh_record.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, f32, f32,
                                      f32, vp]
         L.or_mesh_chains.argtypes = [vp, vp, u32, u32, C.c_int, vp, vp, C.c_int]
+        L.or_paths_edges.restype = C.c_int
+        L.or_paths_edges.argtypes = [vp, f32, u32, vp, C.c_int]
+        L.or_paths_transmission.argtypes = [vp, vp, vp, vp]
+        L.or_knife_edge_amp.restype = f32
+        L.or_knife_edge_amp.argtypes = [f32, f32]
+        L.or_knife_edge_bands.argtypes = [f32, vp]
+        L.or_bend_point.argtypes = [vp, vp, vp, vp, vp]
+        L.or_paths_find.restype = C.c_int
+        L.or_paths_find.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, C.c_int]
         _LIB = L
     return _LIB
 
@@ -269,6 +278,62 @@ class Mesh:
         n = np.zeros(3, np.float32)
         lib().or_mesh_tri_normal(self.ptr, tri, _p(n))
         return n
+
+    def path_edges(self, budget, min_dihedral_deg=40.0):
+        """AutoPaths::new's edge graph (mesh.rs:639-682): [n][7] rows a, b, importance."""
+        cap = max(1, 3 * self.indices.shape[0])
+        out = np.zeros((cap, 7), np.float32)
+        n = lib().or_paths_edges(self.ptr, min_dihedral_deg, budget, _p(out), cap)
+        return out[:n].copy()
+
+    def path_transmission(self, a, b):
+        g = np.zeros(3, np.float32)
+        lib().or_paths_transmission(self.ptr, _p(_f32(a)), _p(_f32(b)), _p(g))
+        return g
+
+    def find_paths(self, edges, src, lis, edge_candidates=24, pair_edges=8, max_paths=4,
+                   taps=False):
+        """AutoPaths::find (paths.rs:117-332): list of dicts (key, points,
+        length, gains), the direct path first."""
+        e = np.ascontiguousarray(edges, np.float32).reshape(-1, 7)
+        bud = PathBudget(edge_candidates, pair_edges, max_paths, 1 if taps else 0)
+        cap = 1 + max_paths
+        out = (FoundPath * cap)()
+        n = lib().or_paths_find(self.ptr, _p(e), e.shape[0], _p(_f32(src)), _p(_f32(lis)),
+                                C.byref(bud), out, cap)
+        return [path_dict(out[i]) for i in range(n)]
+
+
+class PathBudget(C.Structure):
+    _fields_ = [("edge_candidates", C.c_uint32), ("pair_edges", C.c_uint32),
+                ("max_paths", C.c_uint32), ("taps", C.c_uint32)]
+
+
+class FoundPath(C.Structure):
+    _fields_ = [("key", C.c_uint32), ("n_points", C.c_uint32), ("length", C.c_float),
+                ("gains", C.c_float * 3), ("points", C.c_float * 75)]
+
+
+def path_dict(p):
+    pts = np.ctypeslib.as_array(p.points)[:3 * p.n_points].reshape(-1, 3).copy()
+    return dict(key=p.key, points=pts, length=np.float32(p.length),
+                gains=np.array(p.gains[:], np.float32))
+
+
+def knife_edge_amp(detour, freq):
+    return lib().or_knife_edge_amp(detour, freq)
+
+
+def knife_edge_bands(detour):
+    out = np.zeros(3, np.float32)
+    lib().or_knife_edge_bands(detour, _p(out))
+    return out
+
+
+def bend_point(a, b, src, lis):
+    p = np.zeros(3, np.float32)
+    lib().or_bend_point(_p(_f32(a)), _p(_f32(b)), _p(_f32(src)), _p(_f32(lis)), _p(p))
+    return p
 
 
 # ------------------------------------------------------------------ geometry + tracer
