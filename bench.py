@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-stress", action="store_true", help="skip the 2M-triangle town")
     ap.add_argument("--no-late", action="store_true", help="skip the late-field state workload")
+    ap.add_argument("--no-paths", action="store_true", help="skip the knife-edge path search")
     ap.add_argument("--build-only", action="store_true",
                     help="only the town's scene build, on the host and on the device")
     ap.add_argument("--batch-shape", default="",
@@ -121,6 +122,108 @@ def cpu_baseline(mb, src, lis, n_rays, seed, max_seconds):
                 sample=f"the full tracer workload ({src.shape[0]} sources x {n_rays} rays x "
                        f"{lis.shape[0]} listeners) repeated {reps}x in {dt:.1f} s, "
                        f"one job per source on a {threads}-thread pool, counter RNG mode")
+
+
+def _oracle_mesh(mb):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    soup = mb.soup()
+    pos, idx = O.dedup_tris(soup["tri_verts"])
+    mats = [(list(m.absorption), m.scattering, list(m.transmission)) for m in soup["materials"]]
+    return O.Mesh(pos, idx, soup["tri_material"], mats, 3, soup["tri_surface"], soup["tri_max_edge"])
+
+
+def path_search(ctx, dev, reps=5, cpu_seconds=4.0, want_cpu=True):
+    """Knife-edge path search (AutoPaths::find, crates/omg-core/src/paths.rs:117-332)
+    on two workloads: the reference's mesh benchmark (2.5k triangles, 64
+    edges, default budget {24, 8, 4}, its 1000 queries;
+    crates/omg-app/examples/mesh_bench.rs:81-92) and the city block with the
+    world's graph and budget (512 edges, {48, 32, 4}; crates/omg-scene/src/
+    world.rs:154, :638-642) over 4096 seeded (source, listener) pairs.  One
+    step = one omg_paths_find_async call over every query (direct pass and
+    bent pass), inputs and outputs resident in HBM, timed with HIP events on
+    its stream.  The oracle runs the same queries on one host thread for a
+    bounded sample and its paths are compared with the GPU's bit for bit."""
+    import numpy as np
+    import torch
+    import omg_hip as H
+    from omg_hip import scenes
+    from omg_hip.paths import PATH_DTYPE, PathBudget
+    mb2, bounds, boxes = scenes.city_block(seed=2)
+    s2, l2 = scenes.path_queries(7, 4096, bounds, boxes)
+    cases = [("mesh_bench", *scenes.mesh_bench(1000), 64, PathBudget()),
+             ("city_block_world", mb2, s2, l2, 512, PathBudget(48, 32, 4))]
+    legs = {}
+    for name, mb, src, lis, ebud, budget in cases:
+        scene = H.Scene.from_builder(ctx, mb)
+        ap = H.AutoPaths(scene, ebud)
+        n = src.shape[0]
+        stride = 1 + budget.max_paths
+        ap.set_counting(True)
+        ref_raw, ref_cnt = ap.find_many(src, lis, budget, raw=True)
+        cnt = ap.counters()
+        ap.set_counting(False)
+        d_src = torch.from_numpy(src).to(dev)
+        d_lis = torch.from_numpy(lis).to(dev)
+        d_out = torch.zeros(n * stride * PATH_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        d_n = torch.zeros(n, dtype=torch.int32, device=dev)
+        d_work = torch.zeros(n, dtype=torch.int32, device=dev)
+        stream = torch.cuda.Stream(device=dev)
+
+        def run():
+            ap.find_async(d_src.data_ptr(), d_lis.data_ptr(), n, d_out.data_ptr(), d_n.data_ptr(),
+                          d_work.data_ptr(), budget, stream.cuda_stream)
+
+        for _ in range(2):
+            run()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for e in evs:
+            e[0].record(stream)
+            run()
+            e[1].record(stream)
+        torch.cuda.synchronize()
+        ap.status()
+        ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
+        got = d_out.cpu().numpy().view(PATH_DTYPE).reshape(n, stride)
+        got_n = d_n.cpu().numpy().astype(np.uint32)
+        same = bool(np.array_equal(got_n, ref_cnt)) and all(
+            np.array_equal(got[q, :got_n[q]], ref_raw[q, :got_n[q]]) for q in range(n))
+        blocked = int(np.sum(ref_raw[:, 0]["gains"][:, 1] < 0.25))
+        alg = NODE_BYTES * cnt["aabb"] + PRIM_BYTES * cnt["tri"]
+        leg = {"queries": n, "tris": scene.info()["n_tris"], "edges": ap.edge_count(),
+               "budget": [budget.edge_candidates, budget.pair_edges, budget.max_paths],
+               "blocked": blocked, "paths_returned": int(ref_cnt.sum()),
+               "ms_per_launch": ms, "queries_per_s": n / (ms * 1e-3),
+               "us_per_query": ms * 1e3 / n,
+               "aabb_tests": cnt["aabb"], "tri_tests": cnt["tri"],
+               "alg_bytes_GBs": alg / (ms * 1e-3) / 1e9, "timed_equals_counted": same}
+        if want_cpu:
+            om = _oracle_mesh(mb)
+            edges = om.path_edges(ebud)
+            kw = dict(edge_candidates=budget.edge_candidates, pair_edges=budget.pair_edges,
+                      max_paths=budget.max_paths, taps=budget.taps)
+            k, bad, t0 = 0, 0, time.perf_counter()
+            while k < n and (time.perf_counter() - t0 < cpu_seconds or k < 16):
+                ref = om.find_paths(edges, src[k], lis[k], **kw)
+                g = got[k, :got_n[k]]
+                bad += int(len(ref) != len(g) or any(
+                    r["key"] != int(x["key"]) or not np.array_equal(r["gains"], x["gains"]) or
+                    r["length"] != x["length"] or
+                    not np.array_equal(r["points"], x["points"][:int(x["n_points"])])
+                    for r, x in zip(ref, g)))
+                k += 1
+            dt = time.perf_counter() - t0
+            leg["cpu_baseline"] = {"value": k / dt, "unit": "queries/s", "cores": 1, "kind": "port",
+                                   "us_per_query": dt * 1e6 / k,
+                                   "sample": f"the first {k} queries on one host thread"}
+            leg["oracle_mismatches"] = bad
+            leg["oracle_compared"] = k
+        legs[name] = leg
+        ap.close()
+        scene.close()
+    return legs
 
 
 def batched_vs_plain(ctx, launches=3, fixed_shape=None):
@@ -579,6 +682,8 @@ def main():
         out["stress"] = stress(ctx, dev, fixed_shape=fixed)
     if not args.no_late:
         out["late_field"] = late_field(ctx, dev)
+    if not args.no_paths:
+        out["path_search"] = path_search(ctx, dev, want_cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb

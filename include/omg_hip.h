@@ -677,6 +677,90 @@ int omg_param_block_write_flat(const omg_param_block *pb, const omg_tap *taps, u
 int omg_param_block_read_flat(const float *data, uint32_t n_f32, omg_param_block *pb,
                               omg_tap *taps, uint32_t cap, uint32_t *n_taps);
 
+/* ------------------------------------------------------------ knife-edge paths
+ *
+ * AutoPaths (crates/omg-core/src/paths.rs:62-332): the direct segment
+ * with emergent wall thickness, plus, when its mid band is blocked, bent
+ * paths over the mesh's diffraction edges priced by the Kurze-Anderson
+ * knife edge (crates/omg-core/src/diffraction.rs:27-47).  The edge graph is
+ * extracted on the host once (extract_edges + budget_edges,
+ * crates/omg-core/src/mesh.rs:639-682, 40 degree dihedral threshold) and
+ * kept on the device; queries run many (source, listener) pairs in one
+ * launch.  Three bands at 125, 790 and 5600 Hz (3-band scenes only). */
+#define OMG_PATH_MAX_POINTS 25          /* the silhouette hull: 24 drops + the listener */
+#define OMG_PATH_MAX_EDGES 4096         /* edge graph cap: refused above */
+#define OMG_PATH_MAX_CANDIDATES 256     /* edge_candidates cap: refused above */
+#define OMG_PATH_MAX_PAIR_EDGES 64      /* pair_edges cap: refused above */
+
+/* DiffractionEdge (mesh.rs:626-634): endpoints and static importance
+ * (length x sharpness). */
+typedef struct {
+    float a[3];
+    float importance;
+    float b[3];
+    uint32_t reserved;
+} omg_path_edge;
+/* PathBudget (paths.rs:20-45); the reference's default is {24, 8, 4, 0}. */
+typedef struct {
+    uint32_t edge_candidates;
+    uint32_t pair_edges;
+    uint32_t max_paths;
+    uint32_t taps;              /* strict tap semantics: legs priced past the apex */
+} omg_path_budget;
+/* FoundPath (paths.rs:48-60): key 0 = direct, 1 + edge = single bend,
+ * 100000 + 4096 * e1 + e2 = double bend, 90000 = over the silhouette. */
+typedef struct {
+    uint32_t key;
+    uint32_t n_points;
+    float length;
+    float gains[3];
+    float points[OMG_PATH_MAX_POINTS][3];
+} omg_path;
+
+typedef struct omg_paths omg_paths;
+
+/* extract_edges + budget_edges (mesh.rs:639-682) on the host, no device
+ * needed: writes at most cap edges, importance-descending (equal
+ * importance in ascending vertex-pair order), *n_out = the budgeted count. */
+int omg_paths_extract_edges(const float *positions, uint32_t nv, const uint32_t *indices,
+                            uint32_t nt, float min_dihedral_deg, uint32_t budget,
+                            omg_path_edge *out, uint32_t cap, uint32_t *n_out);
+/* AutoPaths::new (paths.rs:70-77): the scene's budgeted edge graph on the
+ * device.  OMG_ERR_CAP above OMG_PATH_MAX_EDGES edges. */
+int omg_paths_create(omg_scene *scene, uint32_t edge_budget, omg_paths **out);
+/* AutoPaths::edge_count (paths.rs:79-81) and the edges themselves. */
+int omg_paths_edges(omg_paths *paths, omg_path_edge *out, uint32_t cap, uint32_t *n_out);
+/* AutoPaths::transmission (paths.rs:84-113) for n segments a[i] -> b[i]:
+ * gains [n][3]. */
+int omg_paths_transmission(omg_paths *paths, const float *a, const float *b, uint32_t n,
+                           float *gains);
+/* AutoPaths::find (paths.rs:117-332) for n queries: out [n][1 + max_paths]
+ * (the direct path first, then the strongest bent paths), n_paths [n].
+ * Blocks until written.  OMG_ERR_OVERFLOW when a segment crossed more
+ * surfaces than the device keeps (64). */
+int omg_paths_find(omg_paths *paths, const omg_path_budget *budget, const float *src,
+                   const float *lis, uint32_t n, omg_path *out, uint32_t *n_paths);
+/* Device-resident variant: d_src, d_lis [n][3], d_out, d_n_paths and
+ * d_work (n uint32 of scratch) are device pointers; enqueued on `stream`
+ * (NULL = the context's stream) without waiting.  An overflow is reported
+ * by the next omg_paths_status. */
+int omg_paths_find_async(omg_paths *paths, const omg_path_budget *budget, const float *d_src,
+                         const float *d_lis, uint32_t n, omg_path *d_out, uint32_t *d_n_paths,
+                         uint32_t *d_work, void *stream);
+/* Waits for the object's queued work; OMG_ERR_OVERFLOW if any launch since
+ * the last call overflowed, else OMG_OK. */
+int omg_paths_status(omg_paths *paths);
+/* AABB and triangle tests of the traversals of the last omg_paths_find
+ * call made with counting on (omg_paths_set_counting). */
+int omg_paths_set_counting(omg_paths *paths, uint32_t on);
+int omg_paths_counters(omg_paths *paths, omg_counters *counters);
+/* The shared numerics, host-evaluated (the device runs the same code):
+ * knife_edge_amp (diffraction.rs:27-42) and bend_point (paths.rs:411-436). */
+float omg_knife_edge_amp(float detour_m, float freq_hz);
+void omg_bend_point(const float a[3], const float b[3], const float src[3], const float lis[3],
+                    float p[3]);
+void omg_paths_destroy(omg_paths *paths);
+
 #ifdef __cplusplus
 }
 #endif

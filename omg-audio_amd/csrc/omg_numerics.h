@@ -126,6 +126,30 @@ OMG_HD f3 unit_sphere(float u1, float u2)
     return mk(r * c, r * s, z);
 }
 
+// Kurze-Anderson knife-edge amplitude (crates/omg-core/src/diffraction.rs:27-42)
+// for a path detour in metres at one frequency.  tanh, tan, log10 and pow
+// are evaluated in f64 and rounded once to f32, on the host and on the
+// device alike (correctly rounded but for double-rounding cases), instead
+// of the platform's f32 functions.
+// band centre frequencies of the knife-edge kernel (diffraction.rs:15)
+OMG_HD float band_hz(int b) { return b == 0 ? 125.0f : (b == 1 ? 790.0f : 5600.0f); }
+
+OMG_HD float knife_edge_amp(float detour_m, float freq_hz)
+{
+    float n = 2.0f * detour_m * freq_hz / kSpeedOfSound;
+    if (n <= -0.1916f) return 1.0f;
+    float x = sqrtf(2.0f * kPi * fabsf(n));
+    float ratio;
+    if (x < 1e-3f)
+        ratio = 1.0f;
+    else if (n >= 0.0f)
+        ratio = x / (float)tanh((double)x);
+    else
+        ratio = x / (float)tan((double)x);
+    float a_db = 5.0f + 20.0f * (float)log10((double)ratio);
+    return (float)pow(10.0, (double)(-a_db / 20.0f));
+}
+
 OMG_HD uint32_t fnv1a_u32(uint32_t h, uint32_t v)
 {
     h = (h ^ (v & 0xFFu)) * 16777619u;

@@ -434,3 +434,6 @@ from .late import GATE_MAX_AGE, LateField, LateParams, TraceGate  # noqa: E402
 from . import wire  # noqa: E402,F401
 
 __all__ += ["Server", "LateField", "LateParams", "TraceGate", "GATE_MAX_AGE", "wire"]
+from .paths import AutoPaths, PathBudget, extract_edges, knife_edge_amp, bend_point  # noqa: E402
+
+__all__ += ["AutoPaths", "PathBudget", "extract_edges", "knife_edge_amp", "bend_point"]

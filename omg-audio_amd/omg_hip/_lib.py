@@ -55,6 +55,27 @@ class Counters(C.Structure):
                 ("bounce_rays", C.c_uint64)]
 
 
+PATH_MAX_POINTS = 25
+PATH_MAX_EDGES = 4096
+PATH_MAX_CANDIDATES = 256
+PATH_MAX_PAIR_EDGES = 64
+
+
+class PathEdge(C.Structure):
+    _fields_ = [("a", C.c_float * 3), ("importance", C.c_float), ("b", C.c_float * 3),
+                ("reserved", C.c_uint32)]
+
+
+class PathBudget(C.Structure):
+    _fields_ = [("edge_candidates", C.c_uint32), ("pair_edges", C.c_uint32),
+                ("max_paths", C.c_uint32), ("taps", C.c_uint32)]
+
+
+class Path(C.Structure):
+    _fields_ = [("key", C.c_uint32), ("n_points", C.c_uint32), ("length", C.c_float),
+                ("gains", C.c_float * 3), ("points", C.c_float * (3 * PATH_MAX_POINTS))]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("n_tris", C.c_uint32), ("n_prims", C.c_uint32), ("n_nodes", C.c_uint32),
                 ("max_depth", C.c_uint32), ("device_bytes", C.c_uint64)]
@@ -263,6 +284,19 @@ SIGNATURES = [
     ("omg_discover", _i, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, C.POINTER(_u32)]),
     ("omg_solve_batch", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp,
                              _vp]),
+    ("omg_paths_extract_edges", _i, [_vp, _u32, _vp, _u32, _f32, _u32, _vp, _u32,
+                                     C.POINTER(_u32)]),
+    ("omg_paths_create", _i, [_vp, _u32, C.POINTER(_vp)]),
+    ("omg_paths_edges", _i, [_vp, _vp, _u32, C.POINTER(_u32)]),
+    ("omg_paths_transmission", _i, [_vp, _vp, _vp, _u32, _vp]),
+    ("omg_paths_find", _i, [_vp, _vp, _vp, _vp, _u32, _vp, _vp]),
+    ("omg_paths_find_async", _i, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("omg_paths_status", _i, [_vp]),
+    ("omg_paths_set_counting", _i, [_vp, _u32]),
+    ("omg_paths_counters", _i, [_vp, _vp]),
+    ("omg_knife_edge_amp", _f32, [_f32, _f32]),
+    ("omg_bend_point", None, [_vp, _vp, _vp, _vp, _vp]),
+    ("omg_paths_destroy", None, [_vp]),
 ]
 
 _LIB = None

@@ -39,6 +39,25 @@ class SplitMix64:
         return lo + self.next_u64() % (hi - lo + 1)
 
 
+class Xorshift64Star:
+    """Rng (crates/omg-core/src/rng.rs:2-22): xorshift64* (Vigna 2014) with
+    next_f32 = top 24 bits / 2^24, an exact f32."""
+
+    def __init__(self, seed):
+        self.state = max(seed & MASK64, 1)
+
+    def next_u64(self):
+        x = self.state
+        x ^= x >> 12
+        x ^= (x << 25) & MASK64
+        x ^= x >> 27
+        self.state = x
+        return (x * 0x2545F4914F6CDD1D) & MASK64
+
+    def next_f32(self):
+        return np.float32(self.next_u64() >> 40) / np.float32(1 << 24)
+
+
 class MeshBuilder:
     """Triangle-soup builder with the reference's authoring calls (mesh.rs:88-153)."""
 
@@ -486,3 +505,48 @@ def c4_workload(seed=4):
     lis = np.vstack([indoor_points(seed * 100 + 3, 2, boxes),
                      outdoor_points(seed * 100 + 4, 2, bounds, boxes)])
     return mb, src, lis
+
+
+def mesh_bench(n_queries=1000):
+    """The reference's mesh benchmark scene and path queries
+    (crates/omg-app/examples/mesh_bench.rs:15-34, :81-92): nine 8 x 6 x 3 m
+    brick blocks on a 15 m grid, 200 clutter boxes drawn from Rng::new(9),
+    an enclosing 55 x 55 x 12.1 m shell; one source at (4, 3, 1.5) inside the
+    first block, listeners at (20 + i % 7, 20 + i % 5, 1.6).  Returns the
+    builder, sources [n][3] and listeners [n][3]."""
+    brick = MaterialDef([0.02, 0.03, 0.04], 0.12, [0.045, 0.008, 0.0012])   # material.rs:45-49
+    mb = MeshBuilder()
+    m = mb.material(brick)
+    rng = Xorshift64Star(9)
+    f = np.float32
+    for i in range(9):
+        x = f((i % 3) * 15.0)
+        y = f((i // 3) * 15.0)
+        mb.abox((x, y, f(0.0)), (x + f(8.0), y + f(6.0), f(3.0)), m)
+    for _ in range(200):
+        p = (rng.next_f32() * f(40.0), rng.next_f32() * f(40.0), f(0.0))
+        sx = f(0.3) + rng.next_f32()
+        sy = f(0.3) + rng.next_f32()
+        sz = f(0.5) + rng.next_f32() * f(1.5)
+        mb.abox(p, (p[0] + sx, p[1] + sy, p[2] + sz), m)
+    mb.abox((-5.0, -5.0, -0.1), (50.0, 50.0, 12.0), m)
+    lis = np.array([(20.0 + i % 7, 20.0 + i % 5, 1.6) for i in range(n_queries)], np.float32)
+    src = np.tile(np.float32([4.0, 3.0, 1.5]), (n_queries, 1))
+    return mb, src, lis
+
+
+def path_queries(seed, n, bounds, boxes):
+    """n (source, listener) pairs for the path search on a city block: ear-
+    height points in the streets and inside buildings, paired at random."""
+    pts = np.vstack([outdoor_points(seed * 100 + 1, 64, bounds, boxes),
+                     indoor_points(seed * 100 + 2, 64, boxes)])
+    rng = SplitMix64(seed)
+    src, lis = [], []
+    while len(src) < n:
+        i = rng.randint(0, pts.shape[0] - 1)
+        j = rng.randint(0, pts.shape[0] - 1)
+        if i != j:
+            src.append(pts[i])
+            lis.append(pts[j])
+    return np.asarray(src, np.float32), np.asarray(lis, np.float32)
+

@@ -3,9 +3,14 @@
  * diffraction.rs, mesh.rs:622-682, material.rs:59-62), restated in C.
  * TEST INFRASTRUCTURE ONLY: the checker of the device path search.
  *
- * Float expressions keep the reference's evaluation order; the
- * transcendental calls (acosf, tanhf, tanf, log10f, powf) are the C
- * library's, as Rust's f32 methods call the platform libm on Linux.
+ * Float expressions keep the reference's evaluation order.  acosf (edge
+ * extraction) is the C library's, as Rust's f32::acos calls the platform
+ * libm.  The knife-edge kernel's tanh, tan, log10 and pow are evaluated
+ * in f64 and rounded once to f32: a correctly rounded f32 result in all
+ * but double-rounding cases (about 1 in 2^28 arguments), within an ulp of
+ * the platform's f32 functions, and reproducible by the device, whose f64
+ * functions are accurate to a few f64 ulps.  Like the tracer's shared
+ * sine/cosine this replaces the platform libm on both sides.
  *
  * One deliberate difference: extract_edges iterates a HashMap, so the
  * reference's order among edges of equal importance is unspecified; here
@@ -106,11 +111,11 @@ float or_knife_edge_amp(float detour_m, float freq_hz)
     if (x < 1e-3f)
         ratio = 1.0f;
     else if (n >= 0.0f)
-        ratio = x / tanhf(x);
+        ratio = x / (float)tanh((double)x);
     else
-        ratio = x / tanf(x);
-    float a_db = 5.0f + 20.0f * log10f(ratio);
-    return powf(10.0f, -a_db / 20.0f);
+        ratio = x / (float)tan((double)x);
+    float a_db = 5.0f + 20.0f * (float)log10((double)ratio);
+    return (float)pow(10.0, (double)(-a_db / 20.0f));
 }
 
 void or_knife_edge_bands(float detour_m, float out[NB])

@@ -141,3 +141,45 @@ def test_clear_line_returns_only_direct(oracle):
     om = oracle_mesh_from_builder(mb)
     paths = om.find_paths(om.path_edges(64), (0.0, 0.0, 1.5), (10.0, 0.0, 1.5))
     assert len(paths) == 1 and abs(paths[0]["gains"][1] - 1.0) < 1e-6
+
+
+# ---- the library's host side (no device): the same numbers as the oracle
+
+def _lib():
+    import omg_hip as H
+    return H
+
+
+def test_library_knife_edge_and_bend_point_match_oracle(oracle):
+    """The shared numerics the device runs, host-evaluated: bit-identical to
+    the oracle's (both round f64 tanh / tan / log10 / pow once to f32)."""
+    H = _lib()
+    rng = np.random.default_rng(9)
+    detours = np.concatenate([rng.uniform(-0.1, 0.1, 3000), rng.uniform(0.0, 30.0, 3000),
+                              [-0.0917, 0.0, 1e-4, 1e-9]]).astype(np.float32)
+    for d in detours:
+        for f in (125.0, 790.0, 5600.0):
+            assert np.float32(H.knife_edge_amp(float(d), f)) == np.float32(oracle.knife_edge_amp(float(d), f))
+    for _ in range(1000):
+        a, b, s, l = (rng.uniform(-30, 30, 3).astype(np.float32) for _ in range(4))
+        assert np.array_equal(H.bend_point(a, b, s, l), oracle.bend_point(a, b, s, l))
+
+
+def test_library_edge_extraction_matches_oracle(oracle):
+    """omg_paths_extract_edges (host) against the oracle on the reference's
+    test meshes and the city block, budget cuts included."""
+    import oracle as O
+    H = _lib()
+    mb, _, _ = scenes.city_block(seed=2)
+    meshes = [mb]
+    for build in (lambda m, mb: mb.abox((0.0, 0.0, 0.0), (4.0, 3.0, 2.0), m),
+                  lambda m, mb: _building(mb, (0.0, 0.0, 0.0), (8.0, 6.0, 3.0), 0.24, m)):
+        b = scenes.MeshBuilder()
+        build(b.material(BRICK), b)
+        meshes.append(b)
+    for b in meshes:
+        soup = b.soup()
+        pos, idx = O.dedup_tris(soup["tri_verts"])
+        om = oracle_mesh_from_builder(b)
+        for budget in (0, 5, 64, 512, 3000):
+            assert np.array_equal(H.extract_edges(pos, idx, budget), om.path_edges(budget))
