@@ -21,6 +21,13 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=o
          "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 
 
+# Per-source flags.  The path search keeps its per-lane crossing lists and
+# traversal stacks in scratch: promoted to VGPRs (the compiler's default
+# for arrays up to a quarter of the register budget) they are indexed per
+# lane through waterfall loops and the bent pass drops to 3 waves per SIMD.
+EXTRA = {"paths_kernels.hip": ["-mllvm", "-amdgpu-promote-alloca-to-vector-limit=64"]}
+
+
 def sources():
     return [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
@@ -34,7 +41,7 @@ def build(verbose=False, jobs=8):
     for s in sources():
         obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
         objs.append(obj)
-        cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, s), "-o", obj]
+        cmd = [HIPCC, *FLAGS, *EXTRA.get(s, []), "-x", "hip", "-c", os.path.join(CSRC, s), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

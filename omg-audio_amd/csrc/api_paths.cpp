@@ -19,7 +19,7 @@ static_assert(sizeof(omg_path) == sizeof(PathDev), "path layout");
 struct omg_paths {
     omg_scene* scene = nullptr;
     std::vector<omg_path_edge> edges;
-    DevBuf d_edges, d_src, d_lis, d_out, d_n, d_work, d_cnt, d_trav;
+    DevBuf d_edges, d_src, d_lis, d_out, d_n, d_work, d_cnt, d_trav, d_plan;
     bool counting = false;
     omg_counters last{};
 };
@@ -44,6 +44,13 @@ int check_budget(const omg_paths* p, const omg_path_budget* b)
     return OMG_OK;
 }
 
+// The per-query plans of the ranking pass (grow-only, kept by the object).
+hipError_t ensure_plan(omg_paths* p, const omg_path_budget* b, uint32_t n)
+{
+    const PlanLayout pl = plan_layout((uint32_t)p->edges.size(), b->edge_candidates, b->pair_edges);
+    return p->d_plan.ensure((size_t)n * pl.words * sizeof(float));
+}
+
 void fill_args(omg_paths* p, const omg_path_budget* b, PathsArgs& A)
 {
     omg_scene* s = p->scene;
@@ -55,6 +62,7 @@ void fill_args(omg_paths* p, const omg_path_budget* b, PathsArgs& A)
     A.pair_edges = b->pair_edges;
     A.max_paths = b->max_paths;
     A.taps = b->taps ? 1u : 0u;
+    A.plan = p->d_plan.as<float>();
 }
 
 }  // namespace
@@ -241,6 +249,7 @@ extern "C" int omg_paths_find(omg_paths* p, const omg_path_budget* b, const floa
     OMG_HIP_CHECK(p->d_out.ensure((size_t)n * stride * sizeof(PathDev)));
     OMG_HIP_CHECK(p->d_n.ensure((size_t)n * sizeof(uint32_t)));
     OMG_HIP_CHECK(p->d_work.ensure((size_t)n * sizeof(uint32_t)));
+    OMG_HIP_CHECK(ensure_plan(p, b, n));
     OMG_HIP_CHECK(launch_fill_zero(p->d_cnt.as<float>(), 2, c->stream));
     if (p->counting) OMG_HIP_CHECK(launch_fill_zero_u64(p->d_trav.as<unsigned long long>(), 2, c->stream));
     PathsArgs A{};
@@ -280,6 +289,11 @@ extern "C" int omg_paths_find_async(omg_paths* p, const omg_path_budget* b, cons
     omg_ctx* c = p->scene->ctx;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     OMG_HIP_CHECK(hipSetDevice(c->device));
+    if (p->d_plan.cap < (size_t)n * plan_layout((uint32_t)p->edges.size(), b->edge_candidates, b->pair_edges).words * sizeof(float)) {
+        // growing the plan buffer frees the old one: wait for work that may read it
+        OMG_HIP_CHECK(hipDeviceSynchronize());
+        OMG_HIP_CHECK(ensure_plan(p, b, n));
+    }
     // the blocked count is per launch; the overflow flag stays set until
     // omg_paths_status reads it (the count word is cleared by a fill kernel)
     OMG_HIP_CHECK(launch_fill_zero(p->d_cnt.as<float>(), 1, st));
@@ -346,7 +360,7 @@ extern "C" void omg_paths_destroy(omg_paths* p)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* d : {&p->d_edges, &p->d_src, &p->d_lis, &p->d_out, &p->d_n, &p->d_work, &p->d_cnt,
-                      &p->d_trav})
+                      &p->d_trav, &p->d_plan})
         d->release();
     delete p;
     ctx_release(c);

@@ -39,13 +39,6 @@ __device__ __forceinline__ uint64_t desc_key(float v, uint32_t ord)
     return ((uint64_t)(~u) << 32) | ord;
 }
 
-__device__ __forceinline__ uint32_t pow2_at_least(uint32_t n)
-{
-    uint32_t p = 2;
-    while (p < n) p <<= 1;
-    return p;
-}
-
 // Ascending bitonic sort of k[0, n), n a power of two, by the whole block.
 __device__ __forceinline__ void block_sort(uint64_t* k, uint32_t n)
 {
@@ -95,26 +88,126 @@ __device__ __forceinline__ void load_edge(const PathEdgeDev* e, uint32_t i, f3& 
     b = mk(y.x, y.y, y.z);
 }
 
-// AutoPaths::transmission (paths.rs:84-113): crossings paired entry/exit by
-// material give the wall thickness of Material::transmission_at
-// (material.rs:59-62).  Returns -1 when the crossing list overflowed.
-template <bool COUNT>
-__device__ __forceinline__ int transmission(const GeomArgs& G, const omg_material_dev* mats, f3 a, f3 b,
-                                            float g[3], TravCount& tc)
+// One crossing of a segment: its position along it (0..1) and the
+// crossed triangle's surface << 16 | material.
+struct Crossing {
+    float t;
+    uint32_t sm;
+};
+
+// Slab test of one box against the ray interval [tlo, thi] (ray_aabb,
+// mesh.rs:409-426, with t0 starting at tlo): tlo = 0, thi = len is the
+// reference's segment test.
+__device__ __forceinline__ bool ray_range(const float bmin[3], const float bmax[3], f3 o, f3 inv,
+                                          float tlo, float thi)
 {
-    SegHit buf[kPathSegCap];
+    float t0 = tlo, t1 = thi;
+    ray_slab(bmin[0], bmax[0], o.x, inv.x, t0, t1);
+    ray_slab(bmin[1], bmax[1], o.y, inv.y, t0, t1);
+    ray_slab(bmin[2], bmax[2], o.z, inv.z, t0, t1);
+    return !(t0 > t1);
+}
+
+// Mesh::segment_hits' traversal (mesh.rs:546-587) over the ray interval
+// [tlo, thi], keeping the crossings with lo <= t/len < hi: stable insertion
+// by t, so equal positions keep the visiting order.  The walk continues
+// into the right child when its box is hit and pushes the left one only
+// when both are (the reference pushes left then right and pops right
+// first: the same leaves in the same order, with fewer stack operations).
+// With the whole interval ([0, len], lo = -inf, hi = inf) it performs the
+// reference's box and triangle tests exactly.  Returns the count, or -1
+// when more than `cap` crossings were found.
+template <bool COUNT>
+__device__ __forceinline__ int seg_walk(const GeomArgs& G, f3 a, f3 dn, f3 inv, float len, float tlo,
+                                        float thi, float lo, float hi, Crossing* out, int cap,
+                                        TravCount& tc)
+{
+    if (COUNT) tc.aabb++;
+    if (!ray_range(G.root_min, G.root_max, a, inv, tlo, thi)) return 0;
+    uint32_t stack[kStackCap];
+    int sp = 0, n = 0;
+    bool overflow = false;
+    uint32_t cur = G.root_ref;
+    while (true) {
+        if (cur & kLeafBit) {
+            uint32_t start = cur & kLeafStartMask;
+            uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+            for (uint32_t k = start; k < start + cnt_p; k++) {
+                Prim p = load_prim(G.prims, k);
+                if (COUNT) tc.tri++;
+                float t, u, v;
+                if (ray_tri(p, a, dn, t, u, v)) {
+                    float tt = t / len;
+                    if (tt > 1e-4f && tt < 1.0f - 1e-4f && tt >= lo && tt < hi) {
+                        if (n == cap) {
+                            overflow = true;
+                            continue;
+                        }
+                        int j = n;
+                        while (j > 0 && out[j - 1].t > tt) {
+                            out[j] = out[j - 1];
+                            j--;
+                        }
+                        out[j] = Crossing{tt, (p.surf << 16) | (p.mat & 0xFFFFu)};
+                        n++;
+                    }
+                }
+            }
+            if (sp == 0) break;
+            cur = stack[--sp];
+        } else {
+            const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + cur);
+            float4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
+            const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
+            const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
+            if (COUNT) tc.aabb += 2;
+            bool hl = ray_range(lmin, lmax, a, inv, tlo, thi);
+            bool hr = ray_range(rmin, rmax, a, inv, tlo, thi);
+            if (hr) {
+                if (hl) stack[sp++] = __float_as_uint(n0.w);
+                cur = __float_as_uint(n1.w);
+            } else if (hl) {
+                cur = __float_as_uint(n0.w);
+            } else {
+                if (sp == 0) break;
+                cur = stack[--sp];
+            }
+        }
+    }
+    return overflow ? -1 : n;
+}
+
+// segment_hits' same-t dedup (mesh.rs:589-603): within 1e-4 / max(len, 1)
+// of the cluster's first crossing the lowest surface id wins.
+__device__ __forceinline__ int dedup(Crossing* c, int n, float len)
+{
+    float tol = 1e-4f / fmaxf(len, 1.0f);
+    int w = 0;
+    for (int i = 0; i < n; i++) {
+        if (w > 0 && fabsf(c[i].t - c[w - 1].t) < tol) {
+            if ((c[i].sm >> 16) < (c[w - 1].sm >> 16)) c[w - 1].sm = c[i].sm;
+        } else {
+            c[w++] = c[i];
+        }
+    }
+    return w;
+}
+
+// AutoPaths::transmission's pairing (paths.rs:87-112): crossings paired
+// entry/exit by material give the wall thickness of
+// Material::transmission_at (material.rs:59-62).
+__device__ __forceinline__ void pair_gains(const omg_material_dev* mats, const Crossing* c, int n,
+                                           float len, float g[3])
+{
     g[0] = g[1] = g[2] = 1.0f;
-    int n = segment_hits<COUNT>(G, a, b, buf, kPathSegCap, tc);
-    if (n < 0) return -1;
-    float len = length(sub(b, a));
     int i = 0;
     while (i < n) {
         int enter = i;
-        uint32_t m = buf[enter].mat;
+        uint32_t m = c[enter].sm & 0xFFFFu;
         float th;
-        if (i + 1 < n && buf[i + 1].mat == m) {
+        if (i + 1 < n && (c[i + 1].sm & 0xFFFFu) == m) {
             i += 1;
-            th = fmaxf((buf[i].t - buf[enter].t) * len, 0.01f);
+            th = fmaxf((c[i].t - c[enter].t) * len, 0.01f);
         } else {
             th = 0.1f;
         }
@@ -122,6 +215,26 @@ __device__ __forceinline__ int transmission(const GeomArgs& G, const omg_materia
         for (int band = 0; band < 3; band++) g[band] *= fminf(mats[m].transmission[band] * k, 0.9f);
         i += 1;
     }
+}
+
+// AutoPaths::transmission (paths.rs:84-113) of a -> b on one lane.
+// Returns -1 when the crossing list overflowed.
+template <bool COUNT>
+__device__ __forceinline__ int transmission(const GeomArgs& G, const omg_material_dev* mats, f3 a, f3 b,
+                                            float g[3], TravCount& tc)
+{
+    g[0] = g[1] = g[2] = 1.0f;
+    if (G.empty) return 0;
+    f3 d = sub(b, a);
+    float len = length(d);
+    if (len < 1e-6f) return 0;
+    f3 dn = mul(d, 1.0f / len);
+    f3 inv = mk(1.0f / dn.x, 1.0f / dn.y, 1.0f / dn.z);
+    Crossing buf[kPathSegCap];
+    int n = seg_walk<COUNT>(G, a, dn, inv, len, 0.0f, len, -INFINITY, INFINITY, buf, kPathSegCap, tc);
+    if (n < 0) return -1;
+    n = dedup(buf, n, len);
+    pair_gains(mats, buf, n, len, g);
     return 0;
 }
 
@@ -171,16 +284,95 @@ __device__ __forceinline__ void write_path(PathDev* o, uint32_t key, const f3* p
     }
 }
 
-// The direct segment of every query (paths.rs:124-135).
+// AutoPaths::transmission (paths.rs:84-113) of one segment per group of K
+// lanes, called by every lane of a 64-lane block (QPB = 64 / K groups).
+// Lane k of a group walks only the boxes the ray meets near the interval
+// k/K <= t < (k+1)/K of the segment's parameter and keeps the crossings in
+// it; the groups' lists, concatenated in interval order, are the whole
+// walk's sorted list (equal positions fall in one interval and keep the
+// visiting order), which the group's lane 0 dedups and pairs.  K = 1 is
+// the reference's walk exactly (the counted pass).  Lane 0 gets the gains
+// and returns -1 on overflow; other lanes return 0.
+// Crossings one lane keeps, and a group's merged list: a lane whose
+// interval holds more sends the segment to the one-lane walk, so a group
+// never merges more than K x lane_cap (nor more than the walk's 64).
+constexpr int lane_cap(int K) { return K == 1 ? kPathSegCap : (K >= 8 ? 16 : 8); }
+constexpr int merge_cap(int K) { return K * lane_cap(K) < kPathSegCap ? K * lane_cap(K) : kPathSegCap; }
+
+template <bool COUNT, int K>
+__device__ __forceinline__ int split_transmission(const GeomArgs& G, const omg_material_dev* mats,
+                                                  bool live, f3 a, f3 b, float g[3], TravCount& tc,
+                                                  Crossing (*merged)[merge_cap(K)],
+                                                  uint32_t (*lane_n)[K])
+{
+    constexpr int LANE_CAP = lane_cap(K);
+    constexpr uint32_t kLaneOvf = 0xFFFFFFFFu;
+    const uint32_t gq = threadIdx.x / K, k = threadIdx.x % K;
+    Crossing buf[LANE_CAP];
+    float len = 0.0f;
+    int n = 0;
+    if (live && !G.empty) {
+        f3 d = sub(b, a);
+        len = length(d);
+        if (!(len < 1e-6f)) {
+            f3 dn = mul(d, 1.0f / len);
+            f3 inv = mk(1.0f / dn.x, 1.0f / dn.y, 1.0f / dn.z);
+            float lo = k == 0 ? -INFINITY : (float)k / (float)K;
+            float hi = k == K - 1 ? INFINITY : (float)(k + 1) / (float)K;
+            float margin = 1e-4f * len + 1e-3f;
+            float tlo = k == 0 ? 0.0f : lo * len - margin;
+            float thi = k == K - 1 ? len : hi * len + margin;
+            n = seg_walk<COUNT>(G, a, dn, inv, len, tlo, thi, lo, hi, buf, LANE_CAP, tc);
+        }
+    }
+    lane_n[gq][k] = n < 0 ? kLaneOvf : (uint32_t)n;
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    bool lane_ovf = false;
+    for (int j = 0; j < K; j++) {
+        uint32_t c = lane_n[gq][j];
+        if (c == kLaneOvf) {
+            lane_ovf = true;
+        } else {
+            total += c;
+            if (j < (int)k) off += c;
+        }
+    }
+    if (!lane_ovf && total <= (uint32_t)merge_cap(K))
+        for (int i = 0; i < n; i++) merged[gq][off + i] = buf[i];
+    __syncthreads();
+    g[0] = g[1] = g[2] = 1.0f;
+    if (!live || k != 0) return 0;
+    if (lane_ovf)   // one interval held more crossings than a lane keeps: the whole walk here
+        return transmission<false>(G, mats, a, b, g, tc);
+    if (total > (uint32_t)merge_cap(K)) return -1;   // more than the walk's 64 crossings
+    int m = dedup(merged[gq], (int)total, len);
+    pair_gains(mats, merged[gq], m, len, g);
+    return 0;
+}
+
+// The direct segment of every query (paths.rs:124-135), kDirectSplit lanes
+// per segment (one in the counted pass); blocked queries go to the work list.
 template <bool COUNT>
 __global__ __launch_bounds__(64) void k_paths_direct(PathsArgs A)
 {
-    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int K = COUNT ? 1 : kDirectSplit;
+    constexpr int QPB = 64 / K;
+    __shared__ Crossing merged[QPB][merge_cap(K)];
+    __shared__ uint32_t lane_n[QPB][K];
+    const uint32_t gq = threadIdx.x / K, k = threadIdx.x % K;
+    const uint32_t q = blockIdx.x * QPB + gq;
+    const bool live = q < A.n;
     TravCount tc;
-    if (q < A.n) {
-        f3 s = load3(A.src, q), l = load3(A.lis, q);
-        float g[3];
-        if (transmission<COUNT>(A.g, A.mats, s, l, g, tc) < 0) atomicOr(&A.counters[1], 1u);
+    f3 s = mk(0.0f, 0.0f, 0.0f), l = s;
+    if (live) {
+        s = load3(A.src, q);
+        l = load3(A.lis, q);
+    }
+    float g[3];
+    int rc = split_transmission<COUNT, K>(A.g, A.mats, live, s, l, g, tc, merged, lane_n);
+    if (live && k == 0) {
+        if (rc < 0) atomicOr(&A.counters[1], 1u);
         f3 pts[2] = {s, l};
         write_path(A.out + (size_t)q * (1 + A.max_paths), 0u, pts, 2, length(sub(l, s)), g);
         A.n_out[q] = 1;
@@ -192,26 +384,27 @@ __global__ __launch_bounds__(64) void k_paths_direct(PathsArgs A)
     if (COUNT) flush_counts(A, tc);
 }
 
-// One blocked query per workgroup (paths.rs:136-331).
-template <bool COUNT>
-__global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
+// Ranking pass, one blocked query per workgroup (paths.rs:136-279): the
+// single-bend and pair rankings, the ranked candidates' bend points and
+// the endpoints of every leg to price, written to the query's plan.  The
+// bend-point chains are serial and VALU-bound: 8 waves per SIMD (64 VGPRs)
+// keep more of them in flight.
+__global__ __launch_bounds__(kPathBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_paths_rank(PathsArgs A)
 {
-    extern __shared__ uint64_t sk[];                                   // sort keys
-    __shared__ uint32_t pre[kPathPrefilter];                           // single-bend candidates
-    __shared__ uint32_t near_s[kPathMaxPairEdges], near_l[kPathMaxPairEdges];
-    __shared__ uint32_t sel1[kPathMaxCandidates];                      // ranked single edges
-    __shared__ float sp[kPathMaxCandidates][3];                        // their bend points
-    __shared__ float sl[kPathMaxCandidates][6];                        // leg gains l1, l2
-    __shared__ uint32_t pe[kPathPairKeep][2];                          // ranked pairs (ei, ej)
-    __shared__ float pp[kPathPairKeep][6];                             // p1, p2
-    __shared__ float pl[kPathPairKeep][9];                             // leg gains l1, lm, l2
-    __shared__ float sil_z[kSilSamples];                               // roof height + 0.03, or -inf
-    __shared__ float bg[kPathMaxCandidates + kPathPairKeep + 1][4];    // gains, length
-    __shared__ uint32_t bkey[kPathMaxCandidates + kPathPairKeep + 1];
-    __shared__ float hull[kPathMaxPoints][3];
-    __shared__ uint32_t s_cnt, s_hull;
-
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint32_t s_cnt;
     if (blockIdx.x >= A.counters[0]) return;
+    const RankLds lay = rank_lds(A.ne, A.edge_candidates, A.pair_edges);
+    uint64_t* sk = reinterpret_cast<uint64_t*>(smem + lay.keys);           // sort keys
+    uint32_t* pre = reinterpret_cast<uint32_t*>(smem + lay.pre);           // single-bend candidates
+    uint32_t* near_s = reinterpret_cast<uint32_t*>(smem + lay.near_s);
+    uint32_t* near_l = reinterpret_cast<uint32_t*>(smem + lay.near_l);
+    uint32_t* sel1 = reinterpret_cast<uint32_t*>(smem + lay.sel);          // ranked single edges
+    uint32_t (*pe)[3] = reinterpret_cast<uint32_t (*)[3]>(smem + lay.pe);  // ranked pairs (ei, ej, position)
+    const PlanLayout pl = plan_layout(A.ne, A.edge_candidates, A.pair_edges);
+    float* F = A.plan + (size_t)blockIdx.x * pl.words;
+    uint32_t* P = reinterpret_cast<uint32_t*>(F);
+
     const uint32_t q = A.work[blockIdx.x];
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
     const f3 src = load3(A.src, q), lis = load3(A.lis, q);
@@ -219,12 +412,11 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
     const float seg_len2 = fmaxf(dot(seg, seg), 1e-6f);
     const float direct_len = length(sub(lis, src));
     const uint32_t ne = A.ne;
-    TravCount tc;
 
-    // ---- 1. single bends (paths.rs:164-189)
+    // ---- single bends (paths.rs:164-189)
     uint32_t npre;
     if (ne > kPathPrefilter) {
-        uint32_t n_pad = pow2_at_least(ne);
+        uint32_t n_pad = pow2_ceil(ne);
         corridor_keys(A, src, seg, seg_len2, 0.0f, 1.0f, sk, n_pad);
         block_sort(sk, n_pad);
         npre = count_valid(sk, kPathPrefilter, &s_cnt);
@@ -234,7 +426,7 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
         for (uint32_t i = tid; i < npre; i += bd) pre[i] = i;
     }
     __syncthreads();
-    uint32_t n_pad = pow2_at_least(npre);
+    uint32_t n_pad = pow2_ceil(npre);
     for (uint32_t i = tid; i < n_pad; i += bd) {
         uint64_t key = kNoKey;
         if (i < npre) {
@@ -242,6 +434,9 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
             float imp;
             load_edge(A.edges, pre[i], a, b, imp);
             f3 p = bend_point(a, b, src, lis);
+            F[pl.csp + 3 * i] = p.x;
+            F[pl.csp + 3 * i + 1] = p.y;
+            F[pl.csp + 3 * i + 2] = p.z;
             float d1 = length(sub(p, src)), d2 = length(sub(lis, p));
             if (!(d1 < 0.2f || d2 < 0.2f)) {
                 float detour = d1 + d2 - direct_len;
@@ -253,13 +448,14 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
     __syncthreads();
     block_sort(sk, n_pad);
     uint32_t n1 = count_valid(sk, n_pad < A.edge_candidates ? n_pad : A.edge_candidates, &s_cnt);
-    for (uint32_t r = tid; r < n1; r += bd) sel1[r] = pre[(uint32_t)sk[r]];
+    // sel1: the ranked candidates' positions (edge = pre[position])
+    for (uint32_t r = tid; r < n1; r += bd) sel1[r] = (uint32_t)sk[r];
     __syncthreads();
 
-    // ---- 2. double bends (paths.rs:233-279)
+    // ---- double bends (paths.rs:233-279)
     uint32_t ns = 0, nl = 0;
     if (ne > 0 && A.pair_edges > 0) {
-        uint32_t n_pad_e = pow2_at_least(ne);
+        uint32_t n_pad_e = pow2_ceil(ne);
         corridor_keys(A, src, seg, seg_len2, 0.0f, 0.65f, sk, n_pad_e);
         block_sort(sk, n_pad_e);
         ns = count_valid(sk, n_pad_e < A.pair_edges ? n_pad_e : A.pair_edges, &s_cnt);
@@ -272,7 +468,7 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
         __syncthreads();
     }
     const uint32_t npair = ns * nl;
-    n_pad = pow2_at_least(npair);
+    n_pad = pow2_ceil(npair);
     for (uint32_t i = tid; i < n_pad; i += bd) {
         uint64_t key = kNoKey;
         if (i < npair) {
@@ -288,6 +484,9 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
                     p2 = bend_point(e2a, e2b, p1, lis);
                     p1 = bend_point(e1a, e1b, src, p2);
                 }
+                float* cp = F + pl.cpp + 6 * i;
+                cp[0] = p1.x; cp[1] = p1.y; cp[2] = p1.z;
+                cp[3] = p2.x; cp[4] = p2.y; cp[5] = p2.z;
                 float d1 = length(sub(p1, src)), dm = length(sub(p2, p1)), d2 = length(sub(lis, p2));
                 if (!(d1 < 0.2f || dm < 0.2f || d2 < 0.2f)) {
                     float det1 = d1 + dm - length(sub(p2, src));
@@ -307,121 +506,175 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
         uint32_t i = (uint32_t)sk[r];
         pe[r][0] = near_s[i / nl];
         pe[r][1] = near_l[i % nl];
+        pe[r][2] = i;
     }
     __syncthreads();
 
-    // bend points of the ranked candidates (recomputed: the same f32 result)
-    for (uint32_t r = tid; r < n1 + n2; r += bd) {
-        if (r < n1) {
-            f3 a, b;
-            float imp;
-            load_edge(A.edges, sel1[r], a, b, imp);
-            f3 p = bend_point(a, b, src, lis);
-            sp[r][0] = p.x;
-            sp[r][1] = p.y;
-            sp[r][2] = p.z;
-        } else {
-            uint32_t k = r - n1;
-            f3 e1a, e1b, e2a, e2b;
-            float imp;
-            load_edge(A.edges, pe[k][0], e1a, e1b, imp);
-            load_edge(A.edges, pe[k][1], e2a, e2b, imp);
-            f3 p2 = mul(add(e2a, e2b), 0.5f);
-            f3 p1 = bend_point(e1a, e1b, src, p2);
-            for (int it = 0; it < 3; it++) {
-                p2 = bend_point(e2a, e2b, p1, lis);
-                p1 = bend_point(e1a, e1b, src, p2);
-            }
-            pp[k][0] = p1.x;
-            pp[k][1] = p1.y;
-            pp[k][2] = p1.z;
-            pp[k][3] = p2.x;
-            pp[k][4] = p2.y;
-            pp[k][5] = p2.z;
-        }
-    }
-    __syncthreads();
-
-    // ---- 3. legs and silhouette drops, one lane each
+    // ---- the plan: the ranked candidates' bend points and their legs
     const f3 flat = mk(lis.x - src.x, lis.y - src.y, 0.0f);
     const float run = length(flat);
     const uint32_t n_sil = run < 1.0f ? 0u : (uint32_t)(kSilSamples - 1);
-    const uint32_t n_tasks = 2 * n1 + 3 * n2 + n_sil;
-    for (uint32_t task = tid; task < n_tasks; task += bd) {
-        if (task < 2 * n1) {
-            uint32_t r = task >> 1, leg = task & 1;
-            f3 p = mk(sp[r][0], sp[r][1], sp[r][2]);
-            f3 e = p;
-            if (A.taps) e = add(p, mul(normalize(sub(p, leg == 0 ? src : lis)), 0.06f));
-            float g[3];
-            int rc = leg == 0 ? transmission<COUNT>(A.g, A.mats, src, e, g, tc)
-                              : transmission<COUNT>(A.g, A.mats, e, lis, g, tc);
-            if (rc < 0) atomicOr(&A.counters[1], 1u);
-            for (int b = 0; b < 3; b++) sl[r][3 * leg + b] = g[b];
-        } else if (task < 2 * n1 + 3 * n2) {
-            uint32_t t2 = task - 2 * n1, r = t2 / 3, leg = t2 % 3;
-            f3 p1 = mk(pp[r][0], pp[r][1], pp[r][2]), p2 = mk(pp[r][3], pp[r][4], pp[r][5]);
-            f3 a, b;
-            if (leg == 0) {
-                a = src;
-                b = A.taps ? add(p1, mul(normalize(sub(p1, src)), 0.06f)) : p1;
-            } else if (leg == 1) {
-                a = A.taps ? add(p1, mul(normalize(sub(p1, p2)), 0.06f)) : p1;
-                b = A.taps ? add(p2, mul(normalize(sub(p2, p1)), 0.06f)) : p2;
-            } else {
-                a = A.taps ? add(p2, mul(normalize(sub(p2, lis)), 0.06f)) : p2;
-                b = lis;
-            }
-            float g[3];
-            if (transmission<COUNT>(A.g, A.mats, a, b, g, tc) < 0) atomicOr(&A.counters[1], 1u);
-            for (int k = 0; k < 3; k++) pl[r][3 * leg + k] = g[k];
+    const uint32_t n_legs = 2 * n1 + 3 * n2 + n_sil;
+    auto put = [&](uint32_t leg, f3 a, f3 b) {
+        float* L = F + pl.legs + 6 * leg;
+        L[0] = a.x; L[1] = a.y; L[2] = a.z;
+        L[3] = b.x; L[4] = b.y; L[5] = b.z;
+    };
+    for (uint32_t r = tid; r < n1 + n2 + n_sil; r += bd) {
+        if (r < n1) {
+            const uint32_t i = sel1[r];
+            f3 p = mk(F[pl.csp + 3 * i], F[pl.csp + 3 * i + 1], F[pl.csp + 3 * i + 2]);
+            P[pl.sel + r] = pre[i];
+            F[pl.sp + 3 * r] = p.x;
+            F[pl.sp + 3 * r + 1] = p.y;
+            F[pl.sp + 3 * r + 2] = p.z;
+            f3 e1 = A.taps ? add(p, mul(normalize(sub(p, src)), 0.06f)) : p;
+            f3 e2 = A.taps ? add(p, mul(normalize(sub(p, lis)), 0.06f)) : p;
+            put(2 * r, src, e1);
+            put(2 * r + 1, e2, lis);
+        } else if (r < n1 + n2) {
+            uint32_t k = r - n1;
+            const float* cp = F + pl.cpp + 6 * pe[k][2];
+            f3 p1 = mk(cp[0], cp[1], cp[2]), p2 = mk(cp[3], cp[4], cp[5]);
+            P[pl.pe + 2 * k] = pe[k][0];
+            P[pl.pe + 2 * k + 1] = pe[k][1];
+            float* pp = F + pl.pp + 6 * k;
+            pp[0] = p1.x; pp[1] = p1.y; pp[2] = p1.z;
+            pp[3] = p2.x; pp[4] = p2.y; pp[5] = p2.z;
+            f3 x1 = A.taps ? add(p1, mul(normalize(sub(p1, src)), 0.06f)) : p1;
+            f3 m1 = A.taps ? add(p1, mul(normalize(sub(p1, p2)), 0.06f)) : p1;
+            f3 m2 = A.taps ? add(p2, mul(normalize(sub(p2, p1)), 0.06f)) : p2;
+            f3 x2 = A.taps ? add(p2, mul(normalize(sub(p2, lis)), 0.06f)) : p2;
+            uint32_t base = 2 * n1 + 3 * k;
+            put(base, src, x1);
+            put(base + 1, m1, m2);
+            put(base + 2, x2, lis);
         } else {
-            // over_silhouette's drop i (paths.rs:344-360)
-            uint32_t i = task - 2 * n1 - 3 * n2 + 1;
+            // over_silhouette's drop i (paths.rs:344-350): its origin
+            uint32_t i = r - n1 - n2 + 1;
             float t = (float)i / (float)kSilSamples;
-            float x = src.x + flat.x * t;
-            float y = src.y + flat.y * t;
-            float line_z = src.z + (lis.z - src.z) * t;
-            float th;
+            put(2 * n1 + 3 * n2 + i - 1, mk(src.x + flat.x * t, src.y + flat.y * t, 60.0f),
+                mk(0.0f, 0.0f, -1.0f));
+        }
+    }
+    if (tid == 0) {
+        P[0] = n1;
+        P[1] = n2;
+        P[2] = n_legs;
+        P[3] = n_sil;
+    }
+}
+
+// Leg pass: every leg of every blocked query, K lanes per leg (one in the
+// counted pass); a silhouette drop (paths.rs:350-358) is one downward
+// raycast on the group's first lane.
+template <bool COUNT, int K>
+__global__ __launch_bounds__(64) void k_paths_legs(PathsArgs A)
+{
+    constexpr int QPB = 64 / K;
+    __shared__ Crossing merged[QPB][merge_cap(K)];
+    __shared__ uint32_t lane_n[QPB][K];
+    const PlanLayout pl = plan_layout(A.ne, A.edge_candidates, A.pair_edges);
+    const uint32_t gq = threadIdx.x / K, k = threadIdx.x % K;
+    const uint64_t u = (uint64_t)blockIdx.x * QPB + gq;
+    const uint32_t slot = (uint32_t)(u / pl.max_legs), leg = (uint32_t)(u % pl.max_legs);
+    float* F = A.plan + (size_t)slot * pl.words;
+    const uint32_t* P = reinterpret_cast<const uint32_t*>(F);
+    bool live = false, drop = false;
+    if (slot < A.counters[0]) {
+        uint32_t n_legs = P[2];
+        live = leg < n_legs;
+        drop = live && leg >= n_legs - P[3];
+    }
+    TravCount tc;
+    const float* L = F + pl.legs + 6 * leg;
+    f3 a = mk(0.0f, 0.0f, 0.0f), b = a;
+    if (live) {
+        a = mk(L[0], L[1], L[2]);
+        b = mk(L[3], L[4], L[5]);
+    }
+    float g[3];
+    int rc = split_transmission<COUNT, K>(A.g, A.mats, live && !drop, a, b, g, tc, merged, lane_n);
+    if (live && k == 0) {
+        float* out = F + pl.lg + 3 * leg;
+        if (drop) {
+            float th, z = -INFINITY;
             uint32_t tri, flags;
-            float z = -INFINITY;
-            if (mesh_raycast<COUNT>(A.g, mk(x, y, 60.0f), mk(0.0f, 0.0f, -1.0f), th, tri, flags, tc)) {
+            if (mesh_raycast<COUNT>(A.g, a, b, th, tri, flags, tc)) {
+                // the drop's line height is the segment's at the same t (paths.rs:347-349)
+                uint32_t i = leg - (P[2] - P[3]) + 1;
+                float t = (float)i / (float)kSilSamples;
+                const uint32_t q = A.work[slot];
+                f3 src = load3(A.src, q), lis = load3(A.lis, q);
+                float line_z = src.z + (lis.z - src.z) * t;
                 float top = 60.0f - th;
                 if (top > line_z + 0.05f) z = top + 0.03f;
             }
-            sil_z[i] = z;
+            out[0] = z;
+        } else {
+            if (rc < 0) atomicOr(&A.counters[1], 1u);
+            out[0] = g[0];
+            out[1] = g[1];
+            out[2] = g[2];
         }
     }
-    __syncthreads();
+    if (COUNT) flush_counts(A, tc);
+}
 
-    // ---- 4. assemble the bent paths (paths.rs:190-232, :280-331)
+// Assembly pass, one blocked query per workgroup (paths.rs:190-232,
+// :280-331, :337-405): gains of the priced candidates, the silhouette hull,
+// the final mid-band ranking and the output.
+__global__ __launch_bounds__(kPathBlock) void k_paths_assemble(PathsArgs A)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint32_t s_cnt, s_hull;
+    if (blockIdx.x >= A.counters[0]) return;
+    const AsmLds lay = asm_lds(A.edge_candidates, A.pair_edges);
+    uint64_t* sk = reinterpret_cast<uint64_t*>(smem + lay.keys);
+    float (*bg)[4] = reinterpret_cast<float (*)[4]>(smem + lay.bg);        // gains, length
+    uint32_t* bkey = reinterpret_cast<uint32_t*>(smem + lay.bkey);
+    float (*hull)[3] = reinterpret_cast<float (*)[3]>(smem + lay.hull);
+    const PlanLayout pl = plan_layout(A.ne, A.edge_candidates, A.pair_edges);
+    const float* F = A.plan + (size_t)blockIdx.x * pl.words;
+    const uint32_t* P = reinterpret_cast<const uint32_t*>(F);
+    const uint32_t q = A.work[blockIdx.x];
+    const uint32_t tid = threadIdx.x, bd = blockDim.x;
+    const f3 src = load3(A.src, q), lis = load3(A.lis, q);
+    const float direct_len = length(sub(lis, src));
+    const uint32_t n1 = P[0], n2 = P[1], n_legs = P[2], n_sil = P[3];
+    const float* lg = F + pl.lg;
     const uint32_t nb = n1 + n2 + 1;
     for (uint32_t r = tid; r < n1 + n2; r += bd) {
         bool ok;
         float g[3], len;
         uint32_t key;
         if (r < n1) {
-            f3 p = mk(sp[r][0], sp[r][1], sp[r][2]);
+            f3 p = mk(F[pl.sp + 3 * r], F[pl.sp + 3 * r + 1], F[pl.sp + 3 * r + 2]);
+            const float* l1 = lg + 3 * (2 * r);
+            const float* l2 = l1 + 3;
             float d1 = length(sub(p, src)), d2 = length(sub(lis, p));
-            ok = !(sl[r][0] * sl[r][3] < 1e-4f);
+            ok = !(l1[0] * l2[0] < 1e-4f);
             float detour = d1 + d2 - direct_len;
             for (int b = 0; b < 3; b++)
-                g[b] = knife_edge_amp(fmaxf(detour, 1e-4f), band_hz(b)) * sl[r][b] * sl[r][3 + b];
+                g[b] = knife_edge_amp(fmaxf(detour, 1e-4f), band_hz(b)) * l1[b] * l2[b];
             len = d1 + d2;
-            key = 1u + sel1[r];
+            key = 1u + P[pl.sel + r];
         } else {
             uint32_t k = r - n1;
-            f3 p1 = mk(pp[k][0], pp[k][1], pp[k][2]), p2 = mk(pp[k][3], pp[k][4], pp[k][5]);
+            const float* pp = F + pl.pp + 6 * k;
+            f3 p1 = mk(pp[0], pp[1], pp[2]), p2 = mk(pp[3], pp[4], pp[5]);
+            const float* l1 = lg + 3 * (2 * n1 + 3 * k);
+            const float* lm = l1 + 3;
+            const float* l2 = l1 + 6;
             float d1 = length(sub(p1, src)), dm = length(sub(p2, p1)), d2 = length(sub(lis, p2));
-            ok = !(pl[k][0] * pl[k][3] * pl[k][6] < 1e-4f);
+            ok = !(l1[0] * lm[0] * l2[0] < 1e-4f);
             float det1 = d1 + dm - length(sub(p2, src));
             float det2 = dm + d2 - length(sub(lis, p1));
             for (int b = 0; b < 3; b++)
                 g[b] = knife_edge_amp(fmaxf(det1, 1e-4f), band_hz(b)) *
-                       knife_edge_amp(fmaxf(det2, 1e-4f), band_hz(b)) * pl[k][b] * pl[k][3 + b] *
-                       pl[k][6 + b];
+                       knife_edge_amp(fmaxf(det2, 1e-4f), band_hz(b)) * l1[b] * lm[b] * l2[b];
             len = d1 + dm + d2;
-            key = 100000u + pe[k][0] * 4096u + pe[k][1];
+            key = 100000u + P[pl.pe + 2 * k] * 4096u + P[pl.pe + 2 * k + 1];
         }
         for (int b = 0; b < 3; b++) bg[r][b] = g[b];
         bg[r][3] = len;
@@ -430,20 +683,24 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
     }
     if (tid == bd - 1) {
         // over_silhouette (paths.rs:337-405): upper hull of the drops
-        uint32_t slot = n1 + n2;
+        const uint32_t slot = n1 + n2;
+        const f3 flat = mk(lis.x - src.x, lis.y - src.y, 0.0f);
+        const float run = length(flat);
         uint64_t key = kNoKey;
         s_hull = 0;
         if (n_sil) {
+            const float* drops = lg + 3 * (n_legs - n_sil);
             float ps[kSilSamples + 1], pz[kSilSamples + 1];
             int np = 0;
             ps[np] = 0.0f;
             pz[np] = src.z;
             np++;
             for (int i = 1; i < kSilSamples; i++) {
-                if (sil_z[i] != -INFINITY) {
+                float z = drops[3 * (i - 1)];
+                if (z != -INFINITY) {
                     float t = (float)i / (float)kSilSamples;
                     ps[np] = run * t;
-                    pz[np] = sil_z[i];
+                    pz[np] = z;
                     np++;
                 }
             }
@@ -493,7 +750,7 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
         sk[slot] = key;
     }
     __syncthreads();
-    n_pad = pow2_at_least(nb);
+    uint32_t n_pad = pow2_ceil(nb);
     for (uint32_t i = nb + tid; i < n_pad; i += bd) sk[i] = kNoKey;
     __syncthreads();
     block_sort(sk, n_pad);
@@ -505,14 +762,14 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
         uint32_t np;
         if (r < n1) {
             pts[0] = src;
-            pts[1] = mk(sp[r][0], sp[r][1], sp[r][2]);
+            pts[1] = mk(F[pl.sp + 3 * r], F[pl.sp + 3 * r + 1], F[pl.sp + 3 * r + 2]);
             pts[2] = lis;
             np = 3;
         } else if (r < n1 + n2) {
-            uint32_t k = r - n1;
+            const float* pp = F + pl.pp + 6 * (r - n1);
             pts[0] = src;
-            pts[1] = mk(pp[k][0], pp[k][1], pp[k][2]);
-            pts[2] = mk(pp[k][3], pp[k][4], pp[k][5]);
+            pts[1] = mk(pp[0], pp[1], pp[2]);
+            pts[2] = mk(pp[3], pp[4], pp[5]);
             pts[3] = lis;
             np = 4;
         } else {
@@ -523,7 +780,6 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_bent(PathsArgs A)
         write_path(o + 1 + i, bkey[r], pts, np, bg[r][3], g);
     }
     if (tid == 0) A.n_out[q] = 1 + keep;
-    if (COUNT) flush_counts(A, tc);
 }
 
 // Batched AutoPaths::transmission (paths.rs:84-113), one lane per segment.
@@ -542,19 +798,32 @@ template <bool COUNT>
 hipError_t launch_paths_t(const PathsArgs& A, hipStream_t s)
 {
     if (A.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_paths_direct<COUNT>, dim3((A.n + 63) / 64), dim3(64), 0, s, A);
+    const uint32_t qd = COUNT ? 64u : 64u / kDirectSplit;
+    hipLaunchKernelGGL(k_paths_direct<COUNT>, dim3((A.n + qd - 1) / qd), dim3(64), 0, s, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // the sort buffer holds the largest of: all edges, the prefiltered
-    // candidates, the pairs, the bent paths
-    uint32_t n_keys = A.ne > kPathPrefilter ? A.ne : kPathPrefilter;
-    uint32_t pairs = A.pair_edges * A.pair_edges;
-    if (pairs > n_keys) n_keys = pairs;
-    uint32_t bent = A.edge_candidates + kPathPairKeep + 1;
-    if (bent > n_keys) n_keys = bent;
-    uint32_t n_pad = 2;
-    while (n_pad < n_keys) n_pad <<= 1;
-    hipLaunchKernelGGL(k_paths_bent<COUNT>, dim3(A.n), dim3(kPathBlock), n_pad * sizeof(uint64_t), s, A);
+    const RankLds rl = rank_lds(A.ne, A.edge_candidates, A.pair_edges);
+    hipLaunchKernelGGL(k_paths_rank, dim3(A.n), dim3(kPathBlock), rl.bytes, s, A);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // one group per (query slot, leg); slots past the blocked count exit.
+    // Few legs are split over 8 lanes each (latency); when the legs give
+    // at least 2M lanes at 2 per leg, over 2 (less repeated descent through
+    // the top of the tree).
+    const PlanLayout pl = plan_layout(A.ne, A.edge_candidates, A.pair_edges);
+    const uint64_t units = (uint64_t)A.n * pl.max_legs;
+    const uint32_t K = COUNT ? 1u : (units * kLegSplitMany >= (2ull << 20) ? kLegSplitMany : kLegSplitFew);
+    const uint64_t ql = 64u / K;
+    const uint64_t blocks = (units + ql - 1) / ql;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    if (COUNT)
+        hipLaunchKernelGGL((k_paths_legs<COUNT, 1>), dim3((uint32_t)blocks), dim3(64), 0, s, A);
+    else if (K == kLegSplitMany)
+        hipLaunchKernelGGL((k_paths_legs<COUNT, kLegSplitMany>), dim3((uint32_t)blocks), dim3(64), 0, s, A);
+    else
+        hipLaunchKernelGGL((k_paths_legs<COUNT, kLegSplitFew>), dim3((uint32_t)blocks), dim3(64), 0, s, A);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const AsmLds al = asm_lds(A.edge_candidates, A.pair_edges);
+    hipLaunchKernelGGL(k_paths_assemble, dim3(A.n), dim3(kPathBlock), al.bytes, s, A);
     return hipGetLastError();
 }
 

@@ -182,3 +182,29 @@ def test_counting_leaves_results_unchanged(ctx, city):
     assert np.array_equal(plain[0], counted[0]) and np.array_equal(plain[1], counted[1])
     assert c["aabb"] > 0 and c["tri"] > 0
     ap.close()
+
+
+def test_many_queries_split_legs_match_one_lane_walks(ctx):
+    """4096 city-block queries at the world budget give the leg pass enough
+    legs to split each over 2 lanes (fewer calls split over 8): the result
+    equals the counted pass's one-lane walks for every query and the
+    oracle's for a sample."""
+    mb, bounds, boxes = scenes.city_block(seed=2)
+    scene = H.Scene.from_builder(ctx, mb)
+    om = oracle_mesh_from_builder(mb)
+    src, lis = scenes.path_queries(7, 4096, bounds, boxes)
+    ap = H.AutoPaths(scene, 512)
+    _say("4096 queries, split legs")
+    fast, fast_n = ap.find_many(src, lis, WORLD, raw=True)
+    ap.set_counting(True)
+    ref, ref_n = ap.find_many(src, lis, WORLD, raw=True)
+    assert np.array_equal(fast_n, ref_n)
+    for q in range(src.shape[0]):
+        assert np.array_equal(fast[q, :fast_n[q]], ref[q, :ref_n[q]]), f"query {q}"
+    edges = om.path_edges(512)
+    got = ap.find_many(src[::16], lis[::16], WORLD)
+    for k, q in enumerate(range(0, src.shape[0], 16)):
+        assert_same_paths(got[k], om.find_paths(edges, src[q], lis[q], **_budget_kw(WORLD)),
+                          f"query {q}")
+    ap.close()
+    scene.close()
