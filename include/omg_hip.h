@@ -742,8 +742,10 @@ int omg_paths_find(omg_paths *paths, const omg_path_budget *budget, const float 
                    const float *lis, uint32_t n, omg_path *out, uint32_t *n_paths);
 /* Device-resident variant: d_src, d_lis [n][3], d_out, d_n_paths and
  * d_work (n uint32 of scratch) are device pointers; enqueued on `stream`
- * (NULL = the context's stream) without waiting.  An overflow is reported
- * by the next omg_paths_status. */
+ * (NULL = the context's stream) without waiting.  The object's own
+ * per-query plan buffer and counters are used too, so calls on one object
+ * must not overlap (a growing plan buffer first waits for the device).  An
+ * overflow is reported by the next omg_paths_status. */
 int omg_paths_find_async(omg_paths *paths, const omg_path_budget *budget, const float *d_src,
                          const float *d_lis, uint32_t n, omg_path *d_out, uint32_t *d_n_paths,
                          uint32_t *d_work, void *stream);
