@@ -75,13 +75,13 @@ void fill_args(omg_paths* p, const omg_path_budget* b, PathsArgs& A)
 // sort by importance descending, truncate.  The reference iterates a hash
 // map, so its order among equal importances is unspecified; here it is
 // ascending (min vertex, max vertex).
-extern "C" int omg_paths_extract_edges(const float* positions, uint32_t nv, const uint32_t* indices,
-                                       uint32_t nt, float min_dihedral_deg, uint32_t budget,
-                                       omg_path_edge* out, uint32_t cap, uint32_t* n_out)
+namespace {
+
+// The budgeted edge graph; false on a vertex index out of range.
+bool extract_edges(const float* positions, uint32_t nv, const uint32_t* indices, uint32_t nt,
+                   float min_dihedral_deg, uint32_t budget, std::vector<omg_path_edge>& all)
 {
-    if (!n_out || (nt && (!positions || !indices)) || (cap && !out))
-        return fail(OMG_ERR_INVALID, "bad argument");
-    *n_out = 0;
+    all.clear();
     struct Half {
         uint64_t key;
         uint32_t tri;
@@ -91,7 +91,7 @@ extern "C" int omg_paths_extract_edges(const float* positions, uint32_t nv, cons
     for (uint32_t t = 0; t < nt; t++)
         for (int k = 0; k < 3; k++) {
             uint32_t i = indices[3 * (size_t)t + k], j = indices[3 * (size_t)t + (k + 1) % 3];
-            if (i >= nv || j >= nv) return fail(OMG_ERR_INVALID, "vertex index out of range");
+            if (i >= nv || j >= nv) return false;
             uint32_t lo = std::min(i, j), hi = std::max(i, j);
             h.push_back(Half{(uint64_t)lo << 32 | hi, t});
         }
@@ -104,7 +104,6 @@ extern "C" int omg_paths_extract_edges(const float* positions, uint32_t nv, cons
         f3 c = vert(indices[3 * (size_t)t + 2]);
         return normalize(cross(sub(b, a), sub(c, a)));
     };
-    std::vector<omg_path_edge> all;
     for (size_t s = 0; s < h.size();) {
         size_t e = s;
         while (e < h.size() && h[e].key == h[s].key) e++;
@@ -143,7 +142,23 @@ extern "C" int omg_paths_extract_edges(const float* positions, uint32_t nv, cons
     std::stable_sort(all.begin(), all.end(), [](const omg_path_edge& x, const omg_path_edge& y) {
         return total_key(x.importance) > total_key(y.importance);
     });
-    uint32_t n = (uint32_t)std::min<size_t>(all.size(), budget);
+    if (all.size() > budget) all.resize(budget);
+    return true;
+}
+
+}  // namespace
+
+extern "C" int omg_paths_extract_edges(const float* positions, uint32_t nv, const uint32_t* indices,
+                                       uint32_t nt, float min_dihedral_deg, uint32_t budget,
+                                       omg_path_edge* out, uint32_t cap, uint32_t* n_out)
+{
+    if (!n_out || (nt && (!positions || !indices)) || (cap && !out))
+        return fail(OMG_ERR_INVALID, "bad argument");
+    *n_out = 0;
+    std::vector<omg_path_edge> all;
+    if (!extract_edges(positions, nv, indices, nt, min_dihedral_deg, budget, all))
+        return fail(OMG_ERR_INVALID, "vertex index out of range");
+    uint32_t n = (uint32_t)all.size();
     *n_out = n;
     if (out && cap) std::memcpy(out, all.data(), std::min(n, cap) * sizeof(omg_path_edge));
     return n > cap ? fail(OMG_ERR_CAP, "edge buffer too small") : OMG_OK;
@@ -156,16 +171,16 @@ extern "C" int omg_paths_create(omg_scene* s, uint32_t edge_budget, omg_paths** 
     if (s->nbands != 3) return fail(OMG_ERR_INVALID, "the path search prices 3 bands");
     if (int rc = ensure_host_mesh(s)) return rc;
     uint32_t nv = (uint32_t)(s->positions.size() / 3);
-    uint32_t n = 0;
-    int rc = omg_paths_extract_edges(s->positions.data(), nv, s->indices.data(), s->n_tris, 40.0f,
-                                     edge_budget, nullptr, 0, &n);
-    if (rc && rc != OMG_ERR_CAP) return rc;
-    if (n > OMG_PATH_MAX_EDGES) return fail(OMG_ERR_CAP, "edge graph above OMG_PATH_MAX_EDGES");
+    std::vector<omg_path_edge> edges;
+    // AutoPaths::new's graph: a 40 degree crease threshold (paths.rs:72-77)
+    if (!extract_edges(s->positions.data(), nv, s->indices.data(), s->n_tris, 40.0f, edge_budget, edges))
+        return fail(OMG_ERR_INVALID, "vertex index out of range");
+    if (edges.size() > OMG_PATH_MAX_EDGES) return fail(OMG_ERR_CAP, "edge graph above OMG_PATH_MAX_EDGES");
+    const uint32_t n = (uint32_t)edges.size();
     omg_paths* p = new omg_paths();
     p->scene = s;
-    p->edges.resize(n);
-    rc = omg_paths_extract_edges(s->positions.data(), nv, s->indices.data(), s->n_tris, 40.0f,
-                                 edge_budget, p->edges.data(), n, &n);
+    p->edges = std::move(edges);
+    int rc = OMG_OK;
     omg_ctx* c = s->ctx;
     if (!rc) {
         hipError_t e = hipSetDevice(c->device);

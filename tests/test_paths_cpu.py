@@ -183,3 +183,24 @@ def test_library_edge_extraction_matches_oracle(oracle):
         om = oracle_mesh_from_builder(b)
         for budget in (0, 5, 64, 512, 3000):
             assert np.array_equal(H.extract_edges(pos, idx, budget), om.path_edges(budget))
+
+
+def test_library_edge_extraction_refusals():
+    """A vertex index out of range is refused; a short output buffer gets
+    the top of the graph and OMG_ERR_CAP with the full count."""
+    import ctypes as C
+    from omg_hip import _lib
+    from omg_hip.paths import EDGE_DTYPE
+    L = _lib.lib()
+    pos = np.float32([[0, 0, 0], [3, 0, 0], [0, 4, 0]])
+    bad = np.uint32([[0, 1, 7]])
+    n = C.c_uint32()
+    out = np.zeros(8, EDGE_DTYPE)
+    rc = L.omg_paths_extract_edges(pos.ctypes.data, 3, bad.ctypes.data, 1, 40.0, 64,
+                                   out.ctypes.data, 8, C.byref(n))
+    assert rc == _lib.ERR_INVALID
+    good = np.uint32([[0, 1, 2]])
+    rc = L.omg_paths_extract_edges(pos.ctypes.data, 3, good.ctypes.data, 1, 40.0, 64,
+                                   out.ctypes.data, 2, C.byref(n))
+    assert rc == _lib.ERR_CAP and n.value == 3
+    assert sorted(np.round(out[:2]["importance"], 4).tolist()) == [4.0, 5.0]
