@@ -726,7 +726,9 @@ int omg_paths_extract_edges(const float *positions, uint32_t nv, const uint32_t 
                             uint32_t nt, float min_dihedral_deg, uint32_t budget,
                             omg_path_edge *out, uint32_t cap, uint32_t *n_out);
 /* AutoPaths::new (paths.rs:70-77): the scene's budgeted edge graph on the
- * device.  OMG_ERR_CAP above OMG_PATH_MAX_EDGES edges. */
+ * device.  OMG_ERR_CAP above OMG_PATH_MAX_EDGES edges.  Queries walk the
+ * scene's BVH: the scene must stay alive while the object is queried (it
+ * may be destroyed before omg_paths_destroy). */
 int omg_paths_create(omg_scene *scene, uint32_t edge_budget, omg_paths **out);
 /* AutoPaths::edge_count (paths.rs:79-81) and the edges themselves. */
 int omg_paths_edges(omg_paths *paths, omg_path_edge *out, uint32_t cap, uint32_t *n_out);

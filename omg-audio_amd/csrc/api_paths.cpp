@@ -17,7 +17,8 @@ static_assert(sizeof(omg_path_edge) == sizeof(PathEdgeDev), "edge layout");
 static_assert(sizeof(omg_path) == sizeof(PathDev), "path layout");
 
 struct omg_paths {
-    omg_scene* scene = nullptr;
+    omg_scene* scene = nullptr;   // must outlive the object's queries
+    omg_ctx* ctx = nullptr;
     std::vector<omg_path_edge> edges;
     DevBuf d_edges, d_src, d_lis, d_out, d_n, d_work, d_cnt, d_trav, d_plan;
     bool counting = false;
@@ -179,6 +180,7 @@ extern "C" int omg_paths_create(omg_scene* s, uint32_t edge_budget, omg_paths** 
     const uint32_t n = (uint32_t)edges.size();
     omg_paths* p = new omg_paths();
     p->scene = s;
+    p->ctx = s->ctx;
     p->edges = std::move(edges);
     int rc = OMG_OK;
     omg_ctx* c = s->ctx;
@@ -371,7 +373,7 @@ extern "C" void omg_bend_point(const float a[3], const float b[3], const float s
 extern "C" void omg_paths_destroy(omg_paths* p)
 {
     if (!p) return;
-    omg_ctx* c = p->scene->ctx;
+    omg_ctx* c = p->ctx;   // the scene may already be gone
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* d : {&p->d_edges, &p->d_src, &p->d_lis, &p->d_out, &p->d_n, &p->d_work, &p->d_cnt,
