@@ -493,18 +493,30 @@ def stress(ctx, dev, launches=2, fixed_shape=None):
     d_bins = torch.empty((n_src, n_lis, nb, 6), dtype=torch.float32, device=dev)
     d_dirs = torch.empty((n_src, n_lis, nb, 3), dtype=torch.float32, device=dev)
     stream = torch.cuda.Stream(device=dev)
-    plain_ms = []
-    for k in range(launches + 1):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        H.trace_world_async(scene, params, d_src.data_ptr(), n_src, d_lis.data_ptr(), n_lis,
-                            d_bins.data_ptr(), d_dirs.data_ptr(), stream=stream.cuda_stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        if k:
-            plain_ms.append(e0.elapsed_time(e1))
+
+    def plain_launches():
+        ms = []
+        for k in range(launches + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            H.trace_world_async(scene, params, d_src.data_ptr(), n_src, d_lis.data_ptr(), n_lis,
+                                d_bins.data_ptr(), d_dirs.data_ptr(), stream=stream.cuda_stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if k:
+                ms.append(e0.elapsed_time(e1))
+        return ms
+
+    # the drop-in as it is (64M rays: the persistent regenerating kernel),
+    # then the single-job kernel alone (routing off)
+    plain_ms = plain_launches()
     got = d_bins.cpu().numpy()
     rel = float(np.max(np.abs(got - counted.bins)) / max(float(np.max(np.abs(counted.bins))), 1e-30))
+    ctx.set_route_rays(0)
+    single_ms = plain_launches()
+    got1 = d_bins.cpu().numpy()
+    rel1 = float(np.max(np.abs(got1 - counted.bins)) / max(float(np.max(np.abs(counted.bins))), 1e-30))
+    ctx.set_route_rays()
     b = H.Batch(ctx, [H.QuerySpec(scene, params, src, lis)])
     if fixed_shape:
         b.set_shape(*fixed_shape)
@@ -524,6 +536,7 @@ def stress(ctx, dev, launches=2, fixed_shape=None):
     br = tc["bounce_rays"]
     alg = NODE_BYTES * tc["aabb_tests"] + PRIM_BYTES * tc["tri_tests"]
     pm = float(np.mean(plain_ms)) * 1e-3
+    sm = float(np.mean(single_ms)) * 1e-3
     bs = float(np.mean(batch_s))
     return {
         "workload": f"town of {info['n_tris']} tris (400 buildings with interiors), "
@@ -536,7 +549,11 @@ def stress(ctx, dev, launches=2, fixed_shape=None):
                              speedup_vs_host=build_s / (build["ms"] * 1e-3)),
         "bounce_rays": br, "aabb_tests": tc["aabb_tests"], "tri_tests": tc["tri_tests"],
         "plain": {"ms": pm * 1e3, "bounce_rays_per_s": br / pm,
-                  "roofline_frac": alg / pm / 1e9 / HBM_PEAK_GBS},
+                  "roofline_frac": alg / pm / 1e9 / HBM_PEAK_GBS,
+                  "kernel": "persistent regenerating (omg_trace_world routes jobs of >= 4M rays)"},
+        "plain_single_job_kernel": {"ms": sm * 1e3, "bounce_rays_per_s": br / sm,
+                                    "roofline_frac": alg / sm / 1e9 / HBM_PEAK_GBS,
+                                    "timed_vs_counted_rel_err": rel1},
         "batched": {"ms": bs * 1e3, "bounce_rays_per_s": br / bs, "shape": shape,
                     "roofline_frac": alg / bs / 1e9 / HBM_PEAK_GBS},
         "timed_vs_counted_rel_err": rel,

@@ -121,6 +121,13 @@ int omg_device_name(omg_ctx *ctx, char *buf, uint32_t len);
  * width 2 (the counters are the reference traversal's). */
 int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds, uint32_t lanes,
                           uint32_t width);
+/* Plain launches (omg_trace_world, omg_trace_world_async, omg_trace_box
+ * excluded) of at least min_rays rays (rays per source x sources) run on
+ * the persistent regenerating kernel of the batched launch instead of the
+ * single-job kernel: the same rays and per-ray records, echograms equal up
+ * to the order of their f32 sums.  Default 4194304; 0 = never.  No
+ * reference counterpart (the reference has one kernel per call). */
+int omg_set_route_rays(omg_ctx *ctx, uint64_t min_rays);
 
 /* ------------------------------------------------------------ scenes */
 

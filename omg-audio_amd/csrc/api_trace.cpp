@@ -59,7 +59,7 @@ static void ctx_teardown(omg_ctx* c)
     for (DevBuf* b : {&c->job_mats, &c->job_ovl, &c->job_src, &c->job_lis, &c->job_bins,
                       &c->job_dirs, &c->job_rec, &c->job_cnt, &c->aux0, &c->aux1, &c->aux2,
                       &c->aux3, &c->aux4, &c->fan_dirs, &c->fan_bins, &c->dome_status,
-                      &c->dome_panels})
+                      &c->dome_panels, &c->route_job, &c->route_items, &c->route_cursor})
         b->release();
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -87,6 +87,13 @@ extern "C" int omg_set_launch_config(omg_ctx* c, uint32_t node_cache, uint32_t s
     c->stack_lds = stack_lds;
     c->lanes = lanes;
     c->width = width;
+    return OMG_OK;
+}
+
+extern "C" int omg_set_route_rays(omg_ctx* c, uint64_t min_rays)
+{
+    if (!c) return fail(OMG_ERR_INVALID, "bad argument");
+    c->route_rays = min_rays;
     return OMG_OK;
 }
 
@@ -408,6 +415,31 @@ int prepare_job(omg_ctx* c, const omg_trace_params* p, const std::vector<MatPara
     return OMG_OK;
 }
 
+// Plain mesh jobs of at least route_rays rays run on the persistent
+// regenerating kernel: one launch over the whole GPU whose lanes take new
+// rays as theirs end, where the single-job kernel would wait for the
+// longest ray of every block.  Same rays, same per-ray records.
+bool routed(const omg_ctx* c, bool mesh, uint64_t rays, bool hits)
+{
+    return mesh && !hits && c->route_rays != 0 && rays >= c->route_rays;
+}
+
+int launch_routed(omg_ctx* c, const TraceArgs& A, uint32_t nbands, bool count, hipStream_t st)
+{
+    // the job view, items and cursor are the context's: wait for async work
+    // that may still read them before they are rewritten or regrown
+    int rc = quiesce(c);
+    if (rc) return rc;
+    const uint64_t items = (uint64_t)((A.n_rays + kBatchItemRays - 1) / kBatchItemRays) * A.n_src;
+    OMG_HIP_CHECK(c->route_job.ensure(sizeof(JobView)));
+    OMG_HIP_CHECK(c->route_items.ensure(items * sizeof(BatchItem)));
+    OMG_HIP_CHECK(c->route_cursor.ensure(sizeof(uint32_t)));
+    OMG_HIP_CHECK(launch_trace_routed(A, nbands, A.width, c->n_cu, c->route_job.as<JobView>(),
+                                      c->route_items.as<BatchItem>(), c->route_cursor.as<uint32_t>(),
+                                      count, st));
+    return OMG_OK;
+}
+
 int run_trace_host(omg_ctx* c, TraceArgs& A, bool mesh, uint32_t nbins, uint32_t nbands,
                    const float* sources, uint32_t n_src, const float* listeners, uint32_t n_lis,
                    float* bins, float* dirs, omg_ray_record* records, omg_counters* counters,
@@ -446,7 +478,12 @@ int run_trace_host(omg_ctx* c, TraceArgs& A, bool mesh, uint32_t nbins, uint32_t
     OMG_HIP_CHECK(launch_fill_zero(A.bins, nb_f, c->stream));
     OMG_HIP_CHECK(launch_fill_zero(A.dirs, nd_f, c->stream));
     if (counters) OMG_HIP_CHECK(launch_fill_zero_u64(A.counters, 3, c->stream));
-    OMG_HIP_CHECK(launch_trace(A, mesh, counters != nullptr, c->stream));
+    if (routed(c, mesh, total, hit_ids != nullptr)) {
+        rc = launch_routed(c, A, nbands, counters != nullptr, c->stream);
+        if (rc) return rc;
+    } else {
+        OMG_HIP_CHECK(launch_trace(A, mesh, counters != nullptr, c->stream));
+    }
     OMG_HIP_CHECK(hipMemcpyAsync(bins, A.bins, nb_f * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     OMG_HIP_CHECK(hipMemcpyAsync(dirs, A.dirs, nd_f * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (records)
@@ -624,6 +661,15 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
     A.counters = nullptr;
     A.ray_base = 0;
     A.ray_count = p->n_rays * n_src;
+    if (routed(c, true, (uint64_t)p->n_rays * n_src, false)) {
+        rc = quiesce(c);
+        if (rc) return rc;
+        OMG_HIP_CHECK(launch_fill_zero(d_bins, nb_f, st));
+        OMG_HIP_CHECK(launch_fill_zero(d_dirs, nd_f, st));
+        rc = launch_routed(c, A, s->nbands, false, st);
+        c->async_inflight = true;
+        return rc;
+    }
     OMG_HIP_CHECK(launch_fill_zero(d_bins, nb_f, st));
     OMG_HIP_CHECK(launch_fill_zero(d_dirs, nd_f, st));
     c->async_inflight = true;

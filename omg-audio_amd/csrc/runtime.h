@@ -84,6 +84,10 @@ struct omg_ctx {
     uint32_t width = 4;                 // BVH width of the plain launch (4 where built, else 2)
     uint64_t quad_below = 1ull << 20;   // launches with fewer rays trace a quad per ray
     uint32_t n_cu = 0;                  // compute units (persistent launches size their grid to it)
+    // plain launches of at least route_rays rays run on the persistent
+    // regenerating kernel (0: never); its job view, items and cursor
+    uint64_t route_rays = 1ull << 22;
+    omg::DevBuf route_job, route_items, route_cursor;
     bool closing = false;
     // the *_async entry points leave kernels in flight that read the staged
     // job tables (job_mats, job_ovl) and dome fan / panels; staging the same

@@ -106,5 +106,11 @@ hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s
 // band count of the batch
 hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape& shape,
                               uint32_t n_cu, bool count, hipStream_t s, uint32_t max_blocks = 0);
+// A plain job run by the persistent regenerating kernel: d_job (one
+// JobView), d_items (ceil(n_rays / 64) x n_src items) and d_cursor (one
+// word) are device memory the caller keeps until the launch has run.
+hipError_t launch_trace_routed(const TraceArgs& A, uint32_t nbm, uint32_t width, uint32_t n_cu,
+                               JobView* d_job, BatchItem* d_items, uint32_t* d_cursor,
+                               bool count, hipStream_t s);
 
 }  // namespace omg

@@ -796,6 +796,51 @@ static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint
                          : launch_batch_l<NBM, COUNT, 16, 1, 2>(B, blocks, s);
 }
 
+// A plain job handed to the persistent kernel: its job view is written by
+// a one-lane kernel (the view travels as a kernel argument, so no host
+// staging outlives the call) and its work items are generated on the
+// device, 64 consecutive rays of one source each, in source-major order.
+__global__ void k_route_job(JobView J, JobView* out)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out = J;
+}
+
+__global__ void k_route_items(BatchItem* items, uint32_t n_items, uint32_t per_src, uint32_t n_rays)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    uint32_t src = i / per_src, first = (i - src * per_src) * kBatchItemRays;
+    uint32_t left = n_rays - first;
+    items[i] = BatchItem{0u, src, first, left < kBatchItemRays ? left : kBatchItemRays};
+}
+
+hipError_t launch_trace_routed(const TraceArgs& A, uint32_t nbm, uint32_t width, uint32_t n_cu,
+                               JobView* d_job, BatchItem* d_items, uint32_t* d_cursor,
+                               bool count, hipStream_t s)
+{
+    const uint32_t per_src = (A.n_rays + kBatchItemRays - 1) / kBatchItemRays;
+    const uint64_t n_items = (uint64_t)per_src * A.n_src;
+    if (n_items == 0) return hipSuccess;
+    if (n_items > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_route_job, dim3(1), dim3(64), 0, s, static_cast<const JobView&>(A), d_job);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_route_items, dim3((uint32_t)((n_items + 255) / 256)), dim3(256), 0, s,
+                       d_items, (uint32_t)n_items, per_src, A.n_rays);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_fill_zero(reinterpret_cast<float*>(d_cursor), 1, s)) != hipSuccess) return e;
+    BatchArgs B{};
+    B.jobs = d_job;
+    B.items = d_items;
+    B.n_items = (uint32_t)n_items;
+    B.next_item = d_cursor;
+    B.counters = A.counters;
+    // the batched kernel's default shape: one lane per ray, 12 LDS stack
+    // entries, 6 waves per SIMD, rays regenerated between bounces
+    BatchShape sh{1u, 12u, 6u, 1u, width};
+    return launch_trace_batch(B, nbm, sh, n_cu, count, s, 0);
+}
+
 // Band capacities compiled for batches: 3, 6 and 8 (a batch runs at the
 // smallest one holding its widest query).
 hipError_t launch_trace_batch(const BatchArgs& B, uint32_t nbm, const BatchShape& sh,

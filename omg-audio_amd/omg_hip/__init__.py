@@ -105,6 +105,11 @@ class Context:
         check(lib().omg_device_name(self.h, buf, 128))
         self.device_name = buf.value.decode()
 
+    def set_route_rays(self, min_rays=1 << 22):
+        """Plain launches of at least min_rays rays run on the persistent
+        regenerating kernel (0 = never; omg_set_route_rays)."""
+        check(lib().omg_set_route_rays(self.h, min_rays))
+
     def set_launch_config(self, node_cache=512, stack_lds=16, lanes=0, width=4):
         """Tracer launch shape: top BVH nodes staged in LDS (64 B units), LDS
         stack depth, lanes per ray (1, 4, or 0 = chosen by launch size), BVH

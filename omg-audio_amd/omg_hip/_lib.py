@@ -284,6 +284,7 @@ SIGNATURES = [
     ("omg_discover", _i, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, C.POINTER(_u32)]),
     ("omg_solve_batch", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp,
                              _vp]),
+    ("omg_set_route_rays", _i, [_vp, C.c_uint64]),
     ("omg_paths_extract_edges", _i, [_vp, _u32, _vp, _u32, _f32, _u32, _vp, _u32,
                                      C.POINTER(_u32)]),
     ("omg_paths_create", _i, [_vp, _u32, C.POINTER(_vp)]),

@@ -221,3 +221,29 @@ def test_batch_refusals(ctx, pool):
     with pytest.raises(H.OmgError):
         b.set_shape(1, 32, 4, regen=0, width=4)   # the 4-wide item walk keeps 16 entries
     b.close()
+
+
+def test_plain_launch_routed_to_the_persistent_kernel(ctx, pool):
+    """omg_trace_world of a job at or above the routing threshold runs on
+    the persistent regenerating kernel: every query of the uneven mix gives
+    the single-job kernel's rays, counters and echograms (binary walk), and
+    on the 4-wide walk every unflagged ray is the same."""
+    qs = _queries(pool)
+    _say(f"case plain routed vs single-job kernel: {len(qs)} queries")
+    ctx.set_route_rays(0)
+    single = [_plain(q) for q in qs]
+    wide1 = [H.trace_world(q.scene, q.params, q.sources, q.listeners, q.overlays, records=True)
+             for q in qs]
+    ctx.set_route_rays(1)
+    try:
+        routed = [_plain(q) for q in qs]
+        wide2 = [H.trace_world(q.scene, q.params, q.sources, q.listeners, q.overlays,
+                               records=True) for q in qs]
+    finally:
+        ctx.set_route_rays()
+    for i in range(len(qs)):
+        _same(routed[i], single[i], f"query {i}")
+        for k in ("aabb_tests", "tri_tests", "bounce_rays"):
+            assert routed[i].counters[k] == single[i].counters[k], (i, k)
+        _same(wide2[i], single[i], f"query {i} 4-wide", wide=True)
+        _same(wide1[i], single[i], f"query {i} 4-wide single-job", wide=True)
