@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-stress", action="store_true", help="skip the 2M-triangle town")
     ap.add_argument("--no-late", action="store_true", help="skip the late-field state workload")
     ap.add_argument("--no-paths", action="store_true", help="skip the knife-edge path search")
+    ap.add_argument("--no-ism", action="store_true", help="skip the image-source discovery and solve")
     ap.add_argument("--build-only", action="store_true",
                     help="only the town's scene build, on the host and on the device")
     ap.add_argument("--batch-shape", default="",
@@ -224,6 +225,69 @@ def path_search(ctx, dev, reps=5, cpu_seconds=4.0, want_cpu=True):
         ap.close()
         scene.close()
     return legs
+
+
+def image_sources(ctx, scene, mb, src, lis, reps=3, cpu_seconds=3.0, want_cpu=True):
+    """Image sources on the BVH (WorldDiscovery / WorldSolve,
+    crates/omg-scene/src/early_world.rs:20-62) on the city block: per
+    listener, chain discovery by the 4096-ray rotating fan (the reference's
+    GPU budget, crates/omg-gpu/src/lib.rs:220) and the solve of every
+    (source, chain) pair for the 16 sources (pt_mesh.rs:185-324).  Timed on
+    the host around the synchronous C calls (inputs and records cross
+    PCIe).  The oracle solves a bounded sample of the same pairs on one
+    host thread and its chains and records are compared with the GPU's."""
+    import numpy as np
+    import omg_hip as H
+    disc = H.GpuWorldDiscovery(scene, n_rays=4096)
+    solver = H.GpuWorldSolve(scene)
+    disc_s, solve_s, pairs, valid = [], [], 0, 0
+    found = []
+    for rep in range(reps + 1):
+        for li in range(lis.shape[0]):
+            t0 = time.perf_counter()
+            ch, od = disc.discover(lis[li], li)
+            t1 = time.perf_counter()
+            rec = solver.solve_batch(src, ch, od, lis[li])
+            t2 = time.perf_counter()
+            if rep:
+                disc_s.append(t1 - t0)
+                solve_s.append(t2 - t1)
+                pairs += src.shape[0] * len(od)
+                valid += int(rec["valid"].sum())
+            elif li == 0:
+                found = (ch, od, rec)
+    leg = {"workload": f"city block, {lis.shape[0]} listeners: 4096-ray chain discovery, "
+                       f"then {src.shape[0]} sources x every chain solved",
+           "discover_ms_per_listener": float(np.mean(disc_s)) * 1e3,
+           "solve_ms_per_listener": float(np.mean(solve_s)) * 1e3,
+           "chains_per_listener": pairs / src.shape[0] / len(solve_s),
+           "pairs_per_s": pairs / float(np.sum(solve_s)),
+           "valid_records_per_listener": valid / len(solve_s)}
+    if want_cpu:
+        om = _oracle_mesh(mb)
+        import oracle as O
+        ch, od, rec = found
+        oc, oo = O.mesh_chains(om, lis[0], 4096, 0)
+        leg["chains_identical"] = bool(np.array_equal(oc, ch) and np.array_equal(oo, od))
+        tab, n_surf = O.surface_table(om)
+        k, bad, t0 = 0, 0, time.perf_counter()
+        n = len(od)
+        while k < src.shape[0] * n and (time.perf_counter() - t0 < cpu_seconds or k < 64):
+            si, ci = divmod(k, n)
+            r = O.mesh_record(om, tab, n_surf, ch[ci][:od[ci]], int(od[ci]), src[si], lis[0])
+            if (r is not None) != bool(rec["valid"][k]) or (
+                    r is not None and (r["delay"] != rec["delay"][k] or
+                                       not np.array_equal(r["dir"], rec["dir"][k]) or
+                                       not np.allclose(r["gains"], rec["gains"][k], rtol=2e-6))):
+                bad += 1
+            k += 1
+        dt = time.perf_counter() - t0
+        leg["cpu_baseline"] = {"value": k / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {k} (source, chain) pairs of listener 0 "
+                                         "on one host thread"}
+        leg["oracle_mismatches"] = bad
+        leg["oracle_compared"] = k
+    return leg
 
 
 def batched_vs_plain(ctx, launches=3, fixed_shape=None):
@@ -701,6 +765,9 @@ def main():
         out["late_field"] = late_field(ctx, dev)
     if not args.no_paths:
         out["path_search"] = path_search(ctx, dev, want_cpu=not args.no_cpu_baseline)
+    if not args.no_ism:
+        out["image_sources"] = image_sources(ctx, scene, mb, src, lis,
+                                             want_cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mb, src, lis, n_rays, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb
