@@ -374,7 +374,11 @@ __global__ __launch_bounds__(64) void k_paths_direct(PathsArgs A)
     if (live && k == 0) {
         if (rc < 0) atomicOr(&A.counters[1], 1u);
         f3 pts[2] = {s, l};
-        write_path(A.out + (size_t)q * (1 + A.max_paths), 0u, pts, 2, length(sub(l, s)), g);
+        PathDev* o = A.out + (size_t)q * (1 + A.max_paths);
+        write_path(o, 0u, pts, 2, length(sub(l, s)), g);
+        // the bent slots start cleared: the assembly pass fills the kept ones
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t i = 1; i <= A.max_paths; i++) write_path(o + i, 0u, pts, 0, 0.0f, zero);
         A.n_out[q] = 1;
         if (g[1] < 0.25f) {
             uint32_t slot = atomicAdd(&A.counters[0], 1u);

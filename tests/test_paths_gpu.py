@@ -173,13 +173,20 @@ def test_prefiltered_and_degenerate_graphs(ctx, city):
 
 
 def test_counting_leaves_results_unchanged(ctx, city):
+    """The counted pass (one lane per segment, the reference's tests) gives
+    the timed passes' paths; slots past a query's path count are zero."""
     scene, om, src, lis = city
     ap = H.AutoPaths(scene, 512)
-    plain = ap.find_many(src, lis, WORLD, raw=True)
+    plain, plain_n = ap.find_many(src, lis, WORLD, raw=True)
     ap.set_counting(True)
-    counted = ap.find_many(src, lis, WORLD, raw=True)
+    counted, counted_n = ap.find_many(src, lis, WORLD, raw=True)
     c = ap.counters()
-    assert np.array_equal(plain[0], counted[0]) and np.array_equal(plain[1], counted[1])
+    assert np.array_equal(plain_n, counted_n)
+    for q in range(src.shape[0]):
+        assert np.array_equal(plain[q, :plain_n[q]], counted[q, :counted_n[q]]), f"query {q}"
+        for raw, n in ((plain, plain_n), (counted, counted_n)):
+            rest = raw[q, n[q]:]
+            assert not np.any(rest["n_points"]) and not np.any(rest["points"]), f"query {q}"
     assert c["aabb"] > 0 and c["tri"] > 0
     ap.close()
 
