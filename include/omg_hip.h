@@ -744,9 +744,10 @@ int omg_paths_edges(omg_paths *paths, omg_path_edge *out, uint32_t cap, uint32_t
 int omg_paths_transmission(omg_paths *paths, const float *a, const float *b, uint32_t n,
                            float *gains);
 /* AutoPaths::find (paths.rs:117-332) for n queries: out [n][1 + max_paths]
- * (the direct path first, then the strongest bent paths), n_paths [n].
- * Blocks until written.  OMG_ERR_OVERFLOW when a segment crossed more
- * surfaces than the device keeps (64). */
+ * (the direct path first, then the strongest bent paths; unused slots are
+ * zero), n_paths [n].  Blocks until written.  OMG_ERR_OVERFLOW when a
+ * segment has more than 64 triangle crossings, counted before crossings
+ * at the same position merge (segment_hits' dedup, mesh.rs:589-603). */
 int omg_paths_find(omg_paths *paths, const omg_path_budget *budget, const float *src,
                    const float *lis, uint32_t n, omg_path *out, uint32_t *n_paths);
 /* Device-resident variant: d_src, d_lis [n][3], d_out, d_n_paths and

@@ -786,16 +786,29 @@ __global__ __launch_bounds__(kPathBlock) void k_paths_assemble(PathsArgs A)
     if (tid == 0) A.n_out[q] = 1 + keep;
 }
 
-// Batched AutoPaths::transmission (paths.rs:84-113), one lane per segment.
+// Batched AutoPaths::transmission (paths.rs:84-113): each segment over
+// kDirectSplit lanes, as the direct pass.
 __global__ __launch_bounds__(64) void k_path_transmission(PathTransArgs A)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
+    constexpr int K = kDirectSplit;
+    constexpr int QPB = 64 / K;
+    __shared__ Crossing merged[QPB][merge_cap(K)];
+    __shared__ uint32_t lane_n[QPB][K];
+    const uint32_t gq = threadIdx.x / K, k = threadIdx.x % K;
+    const uint32_t i = blockIdx.x * QPB + gq;
+    const bool live = i < A.n;
     TravCount tc;
+    f3 a = mk(0.0f, 0.0f, 0.0f), b = a;
+    if (live) {
+        a = load3(A.a, i);
+        b = load3(A.b, i);
+    }
     float g[3];
-    if (transmission<false>(A.g, A.mats, load3(A.a, i), load3(A.b, i), g, tc) < 0)
-        atomicOr(&A.counters[1], 1u);
-    for (int b = 0; b < 3; b++) A.gains[3 * i + b] = g[b];
+    int rc = split_transmission<false, K>(A.g, A.mats, live, a, b, g, tc, merged, lane_n);
+    if (live && k == 0) {
+        if (rc < 0) atomicOr(&A.counters[1], 1u);
+        for (int band = 0; band < 3; band++) A.gains[3 * i + band] = g[band];
+    }
 }
 
 template <bool COUNT>
@@ -841,7 +854,8 @@ hipError_t launch_paths(const PathsArgs& A, bool count, hipStream_t s)
 hipError_t launch_path_transmission(const PathTransArgs& A, hipStream_t s)
 {
     if (A.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_path_transmission, dim3((A.n + 63) / 64), dim3(64), 0, s, A);
+    const uint32_t qpb = 64u / kDirectSplit;
+    hipLaunchKernelGGL(k_path_transmission, dim3((A.n + qpb - 1) / qpb), dim3(64), 0, s, A);
     return hipGetLastError();
 }
 

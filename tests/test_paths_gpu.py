@@ -7,6 +7,7 @@ import sys
 
 import numpy as np
 import pytest
+import torch   # before the HIP library is loaded: torch brings its own HIP runtime
 
 import omg_hip as H
 from omg_hip import scenes
@@ -215,3 +216,73 @@ def test_many_queries_split_legs_match_one_lane_walks(ctx):
                           f"query {q}")
     ap.close()
     scene.close()
+
+
+def _wall_stack(n_walls):
+    """n 2 x 2 m brick walls 0.02 m thick, 0.03 m apart, across x = 10..11
+    m: a segment along x at y = 0.3, z = 1 crosses each twice, inside one
+    triangle of each face (at y = 0 it would run through the faces'
+    diagonals and hit both triangles of every face)."""
+    mb = scenes.MeshBuilder()
+    m = mb.material(BRICK)
+    for i in range(n_walls):
+        x = 10.0 + 0.03 * i
+        mb.abox((x, -1.0, 0.0), (x + 0.02, 1.0, 2.0), m)
+    return mb
+
+
+def test_crossing_list_limits(ctx):
+    """60 crossings inside one lane's interval of a split walk send the
+    segment to the one-lane walk (64 kept), which matches the oracle; 80
+    crossings are refused with OMG_ERR_OVERFLOW (no silent truncation)."""
+    a, b = np.float32([0.0, 0.3, 1.0]), np.float32([20.0, 0.3, 1.0])
+    mb = _wall_stack(30)
+    scene = H.Scene.from_builder(ctx, mb)
+    om = oracle_mesh_from_builder(mb)
+    ap = H.AutoPaths(scene, 0)
+    ts, _, _ = om.segment_hits(a, b)
+    assert len(ts) == 60
+    ref = om.path_transmission(a, b)
+    assert np.array_equal(ap.transmission(a, b), ref)
+    paths = ap.find(a, b)
+    assert np.array_equal(paths[0]["gains"], ref)
+    assert_same_paths(paths, om.find_paths(om.path_edges(0), a, b), "60 crossings")
+    ap.close()
+    scene.close()
+    mb = _wall_stack(40)
+    scene = H.Scene.from_builder(ctx, mb)
+    ap = H.AutoPaths(scene, 0)
+    for call in (lambda: ap.transmission(a, b), lambda: ap.find(a, b)):
+        with pytest.raises(H.OmgError) as e:
+            call()
+        assert e.value.code == H._lib.ERR_OVERFLOW
+    ap.close()
+    scene.close()
+
+
+def test_find_async_matches_find(ctx, city):
+    """omg_paths_find_async on device buffers gives omg_paths_find's paths;
+    omg_paths_status reports no overflow."""
+    from omg_hip.paths import PATH_DTYPE
+    scene, om, src, lis = city
+    ap = H.AutoPaths(scene, 512)
+    ref, ref_n = ap.find_many(src, lis, WORLD, raw=True)
+    dev = torch.device("cuda", 0)
+    n, stride = src.shape[0], 1 + WORLD.max_paths
+    d_src = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
+    d_lis = torch.from_numpy(np.ascontiguousarray(lis)).to(dev)
+    d_out = torch.zeros(n * stride * PATH_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    d_n = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_work = torch.zeros(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    for _ in range(2):
+        ap.find_async(d_src.data_ptr(), d_lis.data_ptr(), n, d_out.data_ptr(), d_n.data_ptr(),
+                      d_work.data_ptr(), WORLD, stream.cuda_stream)
+    torch.cuda.synchronize()
+    ap.status()
+    got = d_out.cpu().numpy().view(PATH_DTYPE).reshape(n, stride)
+    got_n = d_n.cpu().numpy().astype(np.uint32)
+    assert np.array_equal(got_n, ref_n)
+    for q in range(n):
+        assert np.array_equal(got[q], ref[q]), f"query {q}"
+    ap.close()
