@@ -116,8 +116,12 @@ __device__ __forceinline__ bool ray_range(const float bmin[3], const float bmax[
 // first: the same leaves in the same order, with fewer stack operations).
 // With the whole interval ([0, len], lo = -inf, hi = inf) it performs the
 // reference's box and triangle tests exactly.  Returns the count, or -1
-// when more than `cap` crossings were found.
-template <bool COUNT>
+// when more than `cap` crossings were found.  PRIM_STEP: a leaf's
+// primitives take one step each, so the lanes of a wave in leaves and
+// those in inner nodes share every step (faster for the short, many-lane
+// interval walks; the whole-leaf loop is faster for long one- and two-lane
+// walks).  Either way the same tests in the same order.
+template <bool COUNT, bool PRIM_STEP = false>
 __device__ __forceinline__ int seg_walk(const GeomArgs& G, f3 a, f3 dn, f3 inv, float len, float tlo,
                                         float thi, float lo, float hi, Crossing* out, int cap,
                                         TravCount& tc)
@@ -127,13 +131,23 @@ __device__ __forceinline__ int seg_walk(const GeomArgs& G, f3 a, f3 dn, f3 inv, 
     uint32_t stack[kStackCap];
     int sp = 0, n = 0;
     bool overflow = false;
-    uint32_t cur = G.root_ref;
+    // cur: the inner node to test next; [k, kend): the primitives of the
+    // leaf being tested (one per step with PRIM_STEP, else all in one)
+    uint32_t cur = G.root_ref, k = 0, kend = 0;
+    auto enter = [&](uint32_t ref) {
+        if (ref & kLeafBit) {
+            k = ref & kLeafStartMask;
+            kend = k + ((ref >> kLeafCountShift) & 7u);
+        } else {
+            cur = ref;
+        }
+    };
+    enter(cur);
     while (true) {
-        if (cur & kLeafBit) {
-            uint32_t start = cur & kLeafStartMask;
-            uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
-            for (uint32_t k = start; k < start + cnt_p; k++) {
+        if (k < kend) {
+            do {
                 Prim p = load_prim(G.prims, k);
+                k++;
                 if (COUNT) tc.tri++;
                 float t, u, v;
                 if (ray_tri(p, a, dn, t, u, v)) {
@@ -141,37 +155,38 @@ __device__ __forceinline__ int seg_walk(const GeomArgs& G, f3 a, f3 dn, f3 inv, 
                     if (tt > 1e-4f && tt < 1.0f - 1e-4f && tt >= lo && tt < hi) {
                         if (n == cap) {
                             overflow = true;
-                            continue;
+                        } else {
+                            int j = n;
+                            while (j > 0 && out[j - 1].t > tt) {
+                                out[j] = out[j - 1];
+                                j--;
+                            }
+                            out[j] = Crossing{tt, (p.surf << 16) | (p.mat & 0xFFFFu)};
+                            n++;
                         }
-                        int j = n;
-                        while (j > 0 && out[j - 1].t > tt) {
-                            out[j] = out[j - 1];
-                            j--;
-                        }
-                        out[j] = Crossing{tt, (p.surf << 16) | (p.mat & 0xFFFFu)};
-                        n++;
                     }
                 }
-            }
+            } while (!PRIM_STEP && k < kend);
+            if (k < kend) continue;   // the leaf's next primitive, next step
             if (sp == 0) break;
-            cur = stack[--sp];
+            enter(stack[--sp]);
+            continue;
+        }
+        const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + cur);
+        float4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
+        const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
+        const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
+        if (COUNT) tc.aabb += 2;
+        bool hl = ray_range(lmin, lmax, a, inv, tlo, thi);
+        bool hr = ray_range(rmin, rmax, a, inv, tlo, thi);
+        if (hr) {
+            if (hl) stack[sp++] = __float_as_uint(n0.w);
+            enter(__float_as_uint(n1.w));
+        } else if (hl) {
+            enter(__float_as_uint(n0.w));
         } else {
-            const OMG_GLOBAL float4* q = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes) + cur);
-            float4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
-            const float lmin[3] = {n0.x, n0.y, n0.z}, lmax[3] = {n1.x, n1.y, n1.z};
-            const float rmin[3] = {n2.x, n2.y, n2.z}, rmax[3] = {n3.x, n3.y, n3.z};
-            if (COUNT) tc.aabb += 2;
-            bool hl = ray_range(lmin, lmax, a, inv, tlo, thi);
-            bool hr = ray_range(rmin, rmax, a, inv, tlo, thi);
-            if (hr) {
-                if (hl) stack[sp++] = __float_as_uint(n0.w);
-                cur = __float_as_uint(n1.w);
-            } else if (hl) {
-                cur = __float_as_uint(n0.w);
-            } else {
-                if (sp == 0) break;
-                cur = stack[--sp];
-            }
+            if (sp == 0) break;
+            enter(stack[--sp]);
         }
     }
     return overflow ? -1 : n;
@@ -322,7 +337,7 @@ __device__ __forceinline__ int split_transmission(const GeomArgs& G, const omg_m
             float margin = 1e-4f * len + 1e-3f;
             float tlo = k == 0 ? 0.0f : lo * len - margin;
             float thi = k == K - 1 ? len : hi * len + margin;
-            n = seg_walk<COUNT>(G, a, dn, inv, len, tlo, thi, lo, hi, buf, LANE_CAP, tc);
+            n = seg_walk<COUNT, (K >= 8)>(G, a, dn, inv, len, tlo, thi, lo, hi, buf, LANE_CAP, tc);
         }
     }
     lane_n[gq][k] = n < 0 ? kLaneOvf : (uint32_t)n;
