@@ -859,15 +859,9 @@ __device__ __forceinline__ Kid kid_of(const Node2& n, uint32_t node, uint32_t si
     return k;
 }
 
-__device__ __forceinline__ float kid_area(const Kid& k)
-{
-    float dx = k.mx[0] - k.mn[0], dy = k.mx[1] - k.mn[1], dz = k.mx[2] - k.mn[2];
-    return dx * dy + dy * dz + dz * dx;
-}
-
-// One 4-wide node from its paired node: the internal child of largest
-// surface area (first on ties) is replaced by its two children, in place,
-// until four slots are used or only leaves are left (the host collapse).
+// One 4-wide node from its paired node, two binary levels (the host
+// collapse): slots 0-1 hold the left child's two children, or the leaf
+// itself in slot 0 and slot 1 unused; slots 2-3 the right child likewise.
 // Internal children keep their paired-node reference until k_collapse_link.
 __global__ void k_collapse_expand(Collapse4Args A)
 {
@@ -877,41 +871,18 @@ __global__ void k_collapse_expand(Collapse4Args A)
     uint32_t nref = A.queue[j];
     const Node2 n = A.nodes[nref];
     Kid k[4];
-    k[0] = kid_of(n, nref, 0);
-    k[1] = kid_of(n, nref, 1);
-    k[2] = k[1];
-    k[3] = k[1];
-    uint32_t nk = 2;
-    for (int step = 0; step < 2; step++) {
-        int pick = -1;
-        float pa = -1.0f;
 #pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-            if (jj >= (int)nk || (k[jj].ref & kLeafBit)) continue;
-            float a = kid_area(k[jj]);
-            if (a > pa) {
-                pa = a;
-                pick = jj;
-            }
+    for (uint32_t h = 0; h < 2; h++) {
+        const Kid half = kid_of(n, nref, h);
+        if (half.ref & kLeafBit) {
+            k[2 * h] = half;
+            k[2 * h + 1] = half;
+            k[2 * h + 1].ref = kNoChild;
+        } else {
+            const Node2 m = A.nodes[half.ref];
+            k[2 * h] = kid_of(m, half.ref, 0);
+            k[2 * h + 1] = kid_of(m, half.ref, 1);
         }
-        if (pick < 0) break;
-        uint32_t mref = 0;
-#pragma unroll
-        for (int jj = 0; jj < 4; jj++)
-            if (jj == pick) mref = k[jj].ref;
-        const Node2 m = A.nodes[mref];
-        Kid l = kid_of(m, mref, 0), r = kid_of(m, mref, 1);
-        Kid o[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            if (t < pick) o[t] = k[t];
-            else if (t == pick) o[t] = l;
-            else if (t == pick + 1) o[t] = r;
-            else o[t] = k[t > 0 ? t - 1 : 0];
-        }
-#pragma unroll
-        for (int t = 0; t < 4; t++) k[t] = o[t];
-        nk++;
     }
     Node4 out;
     uint32_t ni = 0;
@@ -919,7 +890,7 @@ __global__ void k_collapse_expand(Collapse4Args A)
     for (int t = 0; t < 4; t++) {
         Child4& c = out.c[t];
         c.pad = 0;
-        if (t >= (int)nk) {
+        if (k[t].ref == kNoChild) {
             c.mn[0] = c.mn[1] = c.mn[2] = 0.0f;
             c.mx[0] = c.mx[1] = c.mx[2] = 0.0f;
             c.ref = kNoChild;

@@ -455,46 +455,136 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
 
 // ------------------------------------------------------------------ 4-wide traversal
 //
-// The same tree collapsed into 128 B nodes of up to four children
-// (Node4).  At a node every child box is tested against the current best
-// distance; the hit children are ordered by entry distance (equal entries:
-// the tree's left-to-right order), the nearest becomes the current node and
-// the others are pushed farthest first.  Leaves are tested as in the binary
-// walk.  Every triangle with t < best lies in boxes entered at or before t,
-// so the nearest hit is the binary walk's; the order among hits at exactly
-// equal t can differ and such hits carry the tie flag.
+// The same tree collapsed two binary levels at a time into 128 B nodes
+// (Node4): slots 0-1 are the binary node's left half (its left child's two
+// children, or the left child itself when it is a leaf, slot 1 unused),
+// slots 2-3 its right half.  At a node every slot's box is tested against
+// the current best distance, and the hit slots are visited in the binary
+// walk's order: the half with the nearer box first (the left one on equal
+// entries), and inside a half the nearer slot first (again left on ties).
+// The halves' boxes are not stored: a half's box is the union of its
+// slots' boxes, whose slab entry follows from the slots' per-axis entries
+// (half_entry).  Culling is the binary walk's (`t_enter >= best` at pop,
+// the current best as tmax of the box test), so the 4-wide walk visits the
+// binary walk's leaves in the binary walk's order with the same best
+// distance at every step, and its nearest hit - triangle, distance and the
+// winner among hits at equal distance - is the binary walk's, bit for bit.
+// (A binary node whose children are visited next to each other has its
+// children's boxes tested at the 4-wide node's time, with a best that is
+// not smaller; a child that passes this test but not the binary walk's
+// later one has an entry beyond the best at its pop and is culled there.)
 
-// Sort key of a child: entry distance, then slot.  A missed child sorts last.
+// A slot in visiting order: entry distance, reference, hit.
 struct Cand4 {
     float t;
     uint32_t ref;
-    uint32_t slot;
+    uint32_t hit;
 };
 
-// Field-wise selects: selecting whole structs would pick between two
-// addresses and push the array to scratch memory.
-__device__ __forceinline__ void cand_swap(Cand4& a, Cand4& b)
+__device__ __forceinline__ Cand4 csel(bool c, const Cand4& a, const Cand4& b)
 {
-    bool sw = b.t < a.t || (b.t == a.t && b.slot < a.slot);
-    float ta = sw ? b.t : a.t, tb = sw ? a.t : b.t;
-    uint32_t ra = sw ? b.ref : a.ref, rb = sw ? a.ref : b.ref;
-    uint32_t sa = sw ? b.slot : a.slot, sb = sw ? a.slot : b.slot;
-    a.t = ta;
-    a.ref = ra;
-    a.slot = sa;
-    b.t = tb;
-    b.ref = rb;
-    b.slot = sb;
+    Cand4 r;
+    r.t = c ? a.t : b.t;
+    r.ref = c ? a.ref : b.ref;
+    r.hit = c ? a.hit : b.hit;
+    return r;
 }
 
-// 5-comparator network: c[0..3] ascending by (t, slot)
-__device__ __forceinline__ void sort4(Cand4 c[4])
+// Slab test of one slot box that also returns the per-axis entries (the
+// lesser of the two plane distances, as the slab test takes them).
+__device__ __forceinline__ bool slab_entries(const float4& lo, const float4& hi, f3 o, f3 inv,
+                                             float tmax, float e[3], float& t_enter)
 {
-    cand_swap(c[0], c[1]);
-    cand_swap(c[2], c[3]);
-    cand_swap(c[0], c[2]);
-    cand_swap(c[1], c[3]);
-    cand_swap(c[1], c[2]);
+    const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
+    float t0 = 0.0f, t1 = tmax;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float ta = (bmin[a] - get(o, a)) * get(inv, a), tb = (bmax[a] - get(o, a)) * get(inv, a);
+        const bool sw = ta > tb;
+        const float en = sw ? tb : ta, ex = sw ? ta : tb;
+        e[a] = en;
+        t0 = fmaxf(t0, en);
+        t1 = fminf(t1, ex);
+    }
+    t_enter = t0;
+    return !(t0 > t1);
+}
+
+// Least of two per-axis entries as an entry of the union box.  With a
+// finite 1/d every entry is a number (an overflow gives an infinity) and
+// rounding is monotone, so the union's entry is the least of its parts'.
+// With an infinite 1/d (a zero direction component) an entry is NaN when
+// the origin lies on that box plane; the slab test's fmaxf then ignores
+// it, which is what -inf does, and the union's entry is NaN (ignored)
+// exactly when one of its parts' is.
+__device__ __forceinline__ float union_entry(float a, float b, bool infax)
+{
+    float m = fminf(a, b);
+    if (infax) m = (a != a || b != b) ? -__builtin_inff() : m;
+    return m;
+}
+
+// Entry distance of a half's box from its two slots' per-axis entries.
+__device__ __forceinline__ float half_entry(const float ea[3], const float eb[3], bool infax)
+{
+    float t = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) t = fmaxf(t, union_entry(ea[a], eb[a], infax));
+    return t;
+}
+
+// The binary walk's visiting order of a node's four slots (p[0] first):
+// tl / tr are the halves' entry distances, c[j] the slots'.  Misses keep
+// their place; the walk skips them.
+__device__ __forceinline__ void binary_order(const Cand4 c[4], float tl, float tr, Cand4 p[4])
+{
+    const bool rx = !(tl <= tr), r0 = !(c[0].t <= c[1].t), r1 = !(c[2].t <= c[3].t);
+    const Cand4 ln = csel(r0, c[1], c[0]), lf = csel(r0, c[0], c[1]);
+    const Cand4 rn = csel(r1, c[3], c[2]), rf = csel(r1, c[2], c[3]);
+    p[0] = csel(rx, rn, ln);
+    p[1] = csel(rx, rf, lf);
+    p[2] = csel(rx, ln, rn);
+    p[3] = csel(rx, lf, rf);
+}
+
+// Position of slot j in binary_order's sequence.
+__device__ __forceinline__ uint32_t binary_key(uint32_t j, bool rx, bool r0, bool r1)
+{
+    const uint32_t far_half = (j >> 1) ^ (rx ? 1u : 0u);
+    const uint32_t far_slot = (j & 1u) ^ ((j < 2 ? r0 : r1) ? 1u : 0u);
+    return (far_half << 1) | far_slot;
+}
+
+// Tests the four slots of a node in registers (slot j: box lo[j].xyz ..
+// hi[j].xyz, ref lo[j].w) and puts them in the binary walk's order.
+__device__ __forceinline__ void order_children(const float4 lo[4], const float4 hi[4], f3 o, f3 inv,
+                                               bool infax, float tmax, Cand4 p[4], uint32_t& tested)
+{
+    Cand4 c[4];
+    float e[4][3];
+    tested = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t ref = __float_as_uint(lo[j].w);
+        // an unused slot holds a zero box: its test runs but never counts
+        const bool valid = ref != kNoChild;
+        float te = 0.0f;
+        const bool h = slab_entries(lo[j], hi[j], o, inv, tmax, e[j], te) && valid;
+        tested += valid ? 1u : 0u;
+        c[j].t = te;
+        c[j].ref = ref;
+        c[j].hit = h ? 1u : 0u;
+    }
+    const float ul = half_entry(e[0], e[1], infax), ur = half_entry(e[2], e[3], infax);
+    const float tl = c[1].ref == kNoChild ? c[0].t : ul;
+    const float tr = c[3].ref == kNoChild ? c[2].t : ur;
+    binary_order(c, tl, tr, p);
+}
+
+// True when some component of 1/d is infinite.
+__device__ __forceinline__ bool inf_axis(f3 inv)
+{
+    return isinf(inv.x) || isinf(inv.y) || isinf(inv.z);
 }
 
 struct Lds4Trav {
@@ -513,32 +603,6 @@ struct Quad4Trav {
     uint32_t quad, nq, q;
 };
 
-// Tests the children of one 4-wide node already in registers (child j:
-// box lo[j].xyz..hi[j].xyz, ref lo[j].w) and orders them.
-__device__ __forceinline__ int order_children(const float4 lo[4], const float4 hi[4], f3 o, f3 inv,
-                                              float tmax, Cand4 c[4], uint32_t& tested)
-{
-    const float kMiss = __int_as_float(0x7f800000);
-    int nhit = 0;
-    tested = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        uint32_t ref = __float_as_uint(lo[j].w);
-        const float bmin[3] = {lo[j].x, lo[j].y, lo[j].z}, bmax[3] = {hi[j].x, hi[j].y, hi[j].z};
-        // an empty slot holds a zero box: its test runs but never counts
-        float te = 0.0f;
-        const bool valid = ref != kNoChild;
-        bool h = ray_aabb(bmin, bmax, o, inv, tmax, te) && valid;
-        tested += valid ? 1u : 0u;
-        c[j].t = h ? te : kMiss;
-        c[j].ref = ref;
-        c[j].slot = (uint32_t)j;
-        nhit += h ? 1 : 0;
-    }
-    sort4(c);
-    return nhit;
-}
-
 // Resumable state of the 4-wide walk of one lane: the node in hand, its
 // entry distance, the stack depth, the primitives [pk, pend) of the leaf
 // being tested and the nearest hit so far.
@@ -548,6 +612,7 @@ struct Walk4 {
     float cur_t;
     int sp;
     uint32_t pk, pend;
+    uint32_t infax;
     BestHit H;
 };
 
@@ -555,6 +620,7 @@ struct Walk4 {
 __device__ __forceinline__ bool walk4_begin(const GeomArgs& G, Walk4& W, f3 d)
 {
     W.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    W.infax = inf_axis(W.inv) ? 1u : 0u;
     W.cur = G.root4;
     W.cur_t = 0.0f;
     W.sp = 0;
@@ -598,11 +664,14 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
                     hi[j] = q[2 * j + 1];
                 }
             }
-            Cand4 c[4];
+            Cand4 p[4];
             uint32_t tested;
-            int nhit = order_children(lo, hi, o, W.inv, W.H.found() ? W.H.best : FLT_MAX, c, tested);
+            order_children(lo, hi, o, W.inv, W.infax != 0u, W.H.found() ? W.H.best : FLT_MAX, p,
+                           tested);
             if (COUNT) cnt.aabb += tested;
-            if (nhit > 0) {
+            const bool h0 = p[0].hit != 0u, h1 = p[1].hit != 0u, h2 = p[2].hit != 0u,
+                       h3 = p[3].hit != 0u;
+            if (h0 | h1 | h2 | h3) {
                 auto push = [&](uint32_t ref, float t) {
                     if (W.sp < SK) {
                         L.st_ref[W.sp * L.bs + L.lane] = ref;
@@ -613,11 +682,13 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
                     }
                     W.sp++;
                 };
-                if (nhit > 3) push(c[3].ref, c[3].t);
-                if (nhit > 2) push(c[2].ref, c[2].t);
-                if (nhit > 1) push(c[1].ref, c[1].t);
-                W.cur = c[0].ref;
-                W.cur_t = c[0].t;
+                // every hit slot after the first is pushed, the last one deepest
+                if (h3 & (h0 | h1 | h2)) push(p[3].ref, p[3].t);
+                if (h2 & (h0 | h1)) push(p[2].ref, p[2].t);
+                if (h1 & h0) push(p[1].ref, p[1].t);
+                const Cand4 f = csel(h0, p[0], csel(h1, p[1], csel(h2, p[2], p[3])));
+                W.cur = f.ref;
+                W.cur_t = f.t;
                 return true;
             }
         }
@@ -657,9 +728,34 @@ __device__ __forceinline__ bool mesh_raycast4_lds(const GeomArgs& G, const Lds4T
     return W.H.result(t_out, tri_out, flags);
 }
 
-// The same walk on a DPP quad: lane q loads and tests child q (32 B) and
-// primitive q of a leaf; the four results are broadcast inside the quad and
-// every lane orders them identically.  Lane 0 writes the shared LDS stack.
+// quad_perm moves inside a DPP quad: lane q reads lane q ^ 1 (its partner
+// in the same half of a 4-wide node).
+__device__ __forceinline__ uint32_t qswap(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 1 | (0 << 2) | (3 << 4) | (2 << 6), 0xF, 0xF,
+                                              false);
+}
+__device__ __forceinline__ float qswapf(float x) { return __uint_as_float(qswap(__float_as_uint(x))); }
+
+// Quad form of order_children's half entries: lane q holds slot q's
+// per-axis entries e, its entry te and reference ref; every lane of the
+// quad gets both halves' entry distances.
+__device__ __forceinline__ void quad_half_entries(const float e[3], float te, uint32_t ref, bool infax,
+                                                  float& tl, float& tr)
+{
+    float pe[3] = {qswapf(e[0]), qswapf(e[1]), qswapf(e[2])};
+    const uint32_t pref = qswap(ref);
+    const float u = half_entry(e, pe, infax);
+    // on lanes 0 and 2: the half's entry (its second slot unused: the first slot's)
+    const float th = pref == kNoChild ? te : u;
+    tl = qbf<0>(th);
+    tr = qbf<2>(th);
+}
+
+// The same walk on a DPP quad: lane q loads and tests slot q (32 B) and
+// primitive q of a leaf; the slots' results are broadcast inside the quad
+// and every lane orders them identically.  Lane 0 writes the shared LDS
+// stack.
 template <bool COUNT, int SK>
 __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad4Trav& L, f3 o, f3 d,
                                                    float& t_out, uint32_t& tri_out, uint32_t& flags,
@@ -667,6 +763,7 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
 {
     if (G.empty) return false;
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool infax = inf_axis(inv);
     uint32_t ovf_ref[kStackCap - SK];
     float ovf_t[kStackCap - SK];
     int sp = 0;
@@ -674,7 +771,6 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
     float cur_t = 0.0f;
     BestHit H;
     const uint32_t q = L.q;
-    const float kMiss = __int_as_float(0x7f800000);
     auto push = [&](uint32_t ref, float t) {
         if (sp < SK) {
             if (q == 0) {
@@ -715,29 +811,32 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
                     lo = src[0];
                     hi = src[1];
                 }
-                uint32_t ref = __float_as_uint(lo.w);
-                const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
-                float te = 0.0f;
-                bool h = ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te) &&
-                         ref != kNoChild;
-                float tk = h ? te : kMiss;
+                const uint32_t ref = __float_as_uint(lo.w);
+                float e[3], te = 0.0f;
+                const bool h = slab_entries(lo, hi, o, inv, H.found() ? H.best : FLT_MAX, e, te) &&
+                               ref != kNoChild;
+                float tl, tr;
+                quad_half_entries(e, te, ref, infax, tl, tr);
+                const uint32_t hq = h ? 1u : 0u;
                 Cand4 c[4];
-                c[0] = Cand4{qbf<0>(tk), qb<0>(ref), 0u};
-                c[1] = Cand4{qbf<1>(tk), qb<1>(ref), 1u};
-                c[2] = Cand4{qbf<2>(tk), qb<2>(ref), 2u};
-                c[3] = Cand4{qbf<3>(tk), qb<3>(ref), 3u};
+                c[0] = Cand4{qbf<0>(te), qb<0>(ref), qb<0>(hq)};
+                c[1] = Cand4{qbf<1>(te), qb<1>(ref), qb<1>(hq)};
+                c[2] = Cand4{qbf<2>(te), qb<2>(ref), qb<2>(hq)};
+                c[3] = Cand4{qbf<3>(te), qb<3>(ref), qb<3>(hq)};
                 if (COUNT && q == 0)
                     cnt.aabb += (c[0].ref != kNoChild) + (c[1].ref != kNoChild) +
                                 (c[2].ref != kNoChild) + (c[3].ref != kNoChild);
-                int nhit = (c[0].t != kMiss) + (c[1].t != kMiss) + (c[2].t != kMiss) +
-                           (c[3].t != kMiss);
-                sort4(c);
-                if (nhit > 0) {
-                    if (nhit > 3) push(c[3].ref, c[3].t);
-                    if (nhit > 2) push(c[2].ref, c[2].t);
-                    if (nhit > 1) push(c[1].ref, c[1].t);
-                    cur = c[0].ref;
-                    cur_t = c[0].t;
+                Cand4 p[4];
+                binary_order(c, tl, tr, p);
+                const bool h0 = p[0].hit != 0u, h1 = p[1].hit != 0u, h2 = p[2].hit != 0u,
+                           h3 = p[3].hit != 0u;
+                if (h0 | h1 | h2 | h3) {
+                    if (h3 & (h0 | h1 | h2)) push(p[3].ref, p[3].t);
+                    if (h2 & (h0 | h1)) push(p[2].ref, p[2].t);
+                    if (h1 & h0) push(p[1].ref, p[1].t);
+                    const Cand4 f = csel(h0, p[0], csel(h1, p[1], csel(h2, p[2], p[3])));
+                    cur = f.ref;
+                    cur_t = f.t;
                     continue;
                 }
             }
@@ -757,12 +856,12 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
 
 // The quad walk with a stack deep enough for the whole tree (SK at least the
 // tree's worst-case stack, checked on the host), so nothing spills to
-// scratch and the four lanes push in parallel: lane q ranks its own child
-// among the quad's hits (by entry distance, then slot - the order the
-// sorting walk produces), writes it at stack depth sp + nhit - 1 - rank
-// unless it is the nearest, and the nearest child's reference and distance
-// reach every lane through an OR over the quad (only the nearest lane
-// contributes non-zero bits; node 0 is the root, never a child).
+// scratch and the four lanes push in parallel: lane q finds its slot's
+// place among the quad's hits in the binary walk's order, writes it at
+// stack depth sp + nhit - 1 - rank unless it is the first, and the first
+// slot's reference and distance reach every lane through an OR over the
+// quad (only that lane contributes non-zero bits; node 0 is the root,
+// never a child).
 template <bool COUNT, int SK>
 __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const Quad4Trav& L, f3 o,
                                                         f3 d, float& t_out, uint32_t& tri_out,
@@ -770,6 +869,7 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
 {
     if (G.empty) return false;
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool infax = inf_axis(inv);
     int sp = 0;
     uint32_t cur = G.root4;
     float cur_t = 0.0f;
@@ -804,32 +904,34 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                     lo = src[0];
                     hi = src[1];
                 }
-                uint32_t ref = __float_as_uint(lo.w);
-                const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
-                float te = 0.0f;
-                bool h = ray_aabb(bmin, bmax, o, inv, H.found() ? H.best : FLT_MAX, te) &&
-                         ref != kNoChild;
+                const uint32_t ref = __float_as_uint(lo.w);
+                float e[3], te = 0.0f;
+                const bool h = slab_entries(lo, hi, o, inv, H.found() ? H.best : FLT_MAX, e, te) &&
+                               ref != kNoChild;
                 if (COUNT && q == 0) {
                     uint32_t valid = (uint32_t)(__ballot(ref != kNoChild) >> qshift) & 0xFu;
                     cnt.aabb += __popc(valid);
                 }
+                float tl, tr;
+                quad_half_entries(e, te, ref, infax, tl, tr);
                 const uint32_t hm = (uint32_t)(__ballot(h) >> qshift) & 0xFu;
                 const int nhit = __popc(hm);
                 if (nhit > 0) {
-                    // rank of this lane's child among the quad's hits
                     const float t0 = qbf<0>(te), t1 = qbf<1>(te), t2 = qbf<2>(te), t3 = qbf<3>(te);
-                    int rank = 0;
-                    rank += ((hm & 1u) != 0u) & ((t0 < te) | ((t0 == te) & (0u < q))) ? 1 : 0;
-                    rank += ((hm & 2u) != 0u) & ((t1 < te) | ((t1 == te) & (1u < q))) ? 1 : 0;
-                    rank += ((hm & 4u) != 0u) & ((t2 < te) | ((t2 == te) & (2u < q))) ? 1 : 0;
-                    rank += ((hm & 8u) != 0u) & ((t3 < te) | ((t3 == te) & (3u < q))) ? 1 : 0;
+                    const bool rx = !(tl <= tr), r0 = !(t0 <= t1), r1 = !(t2 <= t3);
+                    // hit slots as bits in visiting order, then this slot's place among them
+                    uint32_t hk = 0;
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++) hk |= ((hm >> j) & 1u) << binary_key(j, rx, r0, r1);
+                    const uint32_t key = binary_key(q, rx, r0, r1);
+                    const int rank = __popc(hk & ((1u << key) - 1u));
                     if (h && rank > 0) {
                         const int pos = sp + nhit - 1 - rank;
                         L.st_ref[pos * L.nq + L.quad] = ref;
                         L.st_t[pos * L.nq + L.quad] = te;
                     }
-                    const bool near = h && rank == 0;
-                    const uint32_t nr = near ? ref : 0u, nt = near ? __float_as_uint(te) : 0u;
+                    const bool first = h && rank == 0;
+                    const uint32_t nr = first ? ref : 0u, nt = first ? __float_as_uint(te) : 0u;
                     cur = qb<0>(nr) | qb<1>(nr) | qb<2>(nr) | qb<3>(nr);
                     cur_t = __uint_as_float(qb<0>(nt) | qb<1>(nt) | qb<2>(nt) | qb<3>(nt));
                     sp += nhit - 1;

@@ -340,11 +340,13 @@ int build_scene(const float* positions, uint32_t nv, const uint32_t* indices, ui
     return 0;
 }
 
-// Collapses the paired binary nodes into 4-wide nodes, breadth-first: an
-// internal child (the one of largest box surface area, first in order on
-// equal areas) is replaced by its two children until four slots are used
-// or only leaves are left.  stack4 is the worst-case traversal stack: the
-// largest sum of (children - 1) along a root-to-leaf path.
+// Collapses the paired binary nodes into 4-wide nodes, breadth-first, two
+// binary levels per node: slots 0-1 hold the binary node's left child (its
+// two children when it is internal, else the leaf in slot 0 and slot 1
+// unused), slots 2-3 its right child likewise.  The fixed halves let the
+// walk rebuild the binary walk's visiting order from the slots alone.
+// stack4 is the worst-case traversal stack: the largest sum of
+// (children - 1) along a root-to-leaf path.
 void collapse4(HostScene& S)
 {
     S.nodes4.clear();
@@ -356,55 +358,38 @@ void collapse4(HostScene& S)
         float mn[3], mx[3];
         uint32_t ref;
     };
-    auto area = [](const Kid& k) {
-        float dx = k.mx[0] - k.mn[0], dy = k.mx[1] - k.mn[1], dz = k.mx[2] - k.mn[2];
-        return dx * dy + dy * dz + dz * dx;
-    };
     auto left = [](const Node2& n) {
         return Kid{{n.lmin[0], n.lmin[1], n.lmin[2]}, {n.lmax[0], n.lmax[1], n.lmax[2]}, n.lref};
     };
     auto right = [](const Node2& n) {
         return Kid{{n.rmin[0], n.rmin[1], n.rmin[2]}, {n.rmax[0], n.rmax[1], n.rmax[2]}, n.rref};
     };
+    const Kid none{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, kNoChild};
     std::vector<uint32_t> queue{S.root_ref};  // binary node of each 4-wide node
     S.nodes4.reserve(S.nodes.size() / 2 + 1);
     for (size_t qi = 0; qi < queue.size(); qi++) {
         const Node2& n = S.nodes[queue[qi]];
         Kid k[4];
-        int nk = 2;
-        k[0] = left(n);
-        k[1] = right(n);
-        while (nk < 4) {
-            int pick = -1;
-            float pa = -1.0f;
-            for (int j = 0; j < nk; j++) {
-                if (k[j].ref & kLeafBit) continue;
-                float a = area(k[j]);
-                if (a > pa) {
-                    pa = a;
-                    pick = j;
-                }
+        const Kid half[2] = {left(n), right(n)};
+        for (int h = 0; h < 2; h++) {
+            if (half[h].ref & kLeafBit) {
+                k[2 * h] = half[h];
+                k[2 * h + 1] = none;
+            } else {
+                const Node2& m = S.nodes[half[h].ref];
+                k[2 * h] = left(m);
+                k[2 * h + 1] = right(m);
             }
-            if (pick < 0) break;
-            const Node2& m = S.nodes[k[pick].ref];
-            for (int j = nk; j > pick + 1; j--) k[j] = k[j - 1];
-            k[pick] = left(m);
-            k[pick + 1] = right(m);
-            nk++;
         }
         Node4 o;
         std::memset(&o, 0, sizeof o);
         for (int j = 0; j < 4; j++) {
             Child4& c = o.c[j];
-            if (j >= nk) {
-                c.ref = kNoChild;
-                continue;
-            }
             for (int a = 0; a < 3; a++) {
                 c.mn[a] = k[j].mn[a];
                 c.mx[a] = k[j].mx[a];
             }
-            if (k[j].ref & kLeafBit) {
+            if (k[j].ref == kNoChild || (k[j].ref & kLeafBit)) {
                 c.ref = k[j].ref;
             } else {
                 c.ref = (uint32_t)queue.size();

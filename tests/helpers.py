@@ -85,3 +85,50 @@ def golden_violations(got, expect):
 def c1_scene():
     """Config 1: 12 x 8 x 4 m shoebox as 12 triangles, seed 1."""
     return scenes.shoebox(seed=1)
+
+
+def assert_tree_matches_oracle(prims, nodes, root, om):
+    """The product's packed primitives [n][12] and paired nodes [n][16]
+    (omg_bvh_build_host / omg_scene_export rows, root reference `root`)
+    are the oracle Mesh's tree: primitives bit for bit in the same order,
+    and walking both trees together, the same child boxes and leaf ranges
+    at every node (boxes compared as values: a zero may carry either sign)."""
+    onodes, oprims = om.export()
+    pv = prims.view(np.uint32)
+    ov = oprims.view(np.uint32)
+    assert pv.shape[0] == ov.shape[0]
+    for a, b in [((0, 1, 2), (0, 1, 2)), ((4, 5, 6), (3, 4, 5)), ((8, 9, 10), (6, 7, 8)),
+                 ((3,), (9,)), ((7,), (10,))]:
+        assert np.array_equal(pv[:, list(a)], ov[:, list(b)])
+    on_u = onodes.view(np.uint32)
+
+    def oref(i):
+        a, b = on_u[i, 6], on_u[i, 7]
+        if a & 0x80000000:
+            return ("leaf", int(a & 0x7FFFFFFF), int(b))
+        return ("node", i)
+
+    def pref(r):
+        if r & 0x80000000:
+            return ("leaf", int(r & 0x0FFFFFFF), int((r >> 28) & 7))
+        return ("node", int(r))
+
+    stack = [(0, root)]
+    seen = 0
+    while stack:
+        oi, r = stack.pop()
+        o = oref(oi)
+        p = pref(r)
+        assert o[0] == p[0]
+        if o[0] == "leaf":
+            assert o[1:] == p[1:]
+            continue
+        seen += 1
+        n = nodes[p[1]]
+        nu = n.view(np.uint32)
+        la, lb = int(on_u[oi, 6]), int(on_u[oi, 7])
+        assert np.array_equal(n[0:3], onodes[la, 0:3]) and np.array_equal(n[4:7], onodes[la, 3:6])
+        assert np.array_equal(n[8:11], onodes[lb, 0:3]) and np.array_equal(n[12:15], onodes[lb, 3:6])
+        stack.append((la, int(nu[3])))
+        stack.append((lb, int(nu[7])))
+    assert seen == nodes.shape[0]

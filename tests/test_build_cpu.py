@@ -9,7 +9,7 @@ import pytest
 
 import oracle as O
 from omg_hip import _lib, scenes
-from helpers import oracle_mesh_from_builder
+from helpers import assert_tree_matches_oracle, oracle_mesh_from_builder
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -73,47 +73,7 @@ def test_scene_build_matches_oracle(which, oracle):
     om = oracle_mesh_from_builder(mb)
     opos, oidx = O.dedup_tris(mb.soup()["tri_verts"])
     assert np.array_equal(pos, opos) and np.array_equal(idx, oidx)
-    onodes, oprims = om.export()
-    # primitives: identical bits and order (a, e1, e2, tri, material)
-    pv = prims.view(np.uint32)
-    ov = oprims.view(np.uint32)
-    assert pv.shape[0] == ov.shape[0]
-    for a, b in [((0, 1, 2), (0, 1, 2)), ((4, 5, 6), (3, 4, 5)), ((8, 9, 10), (6, 7, 8)),
-                 ((3,), (9,)), ((7,), (10,))]:
-        assert np.array_equal(pv[:, list(a)], ov[:, list(b)])
-    # paired nodes: walk both trees together and compare child boxes and leaf ranges
-    on_u = onodes.view(np.uint32)
-
-    def oref(i):
-        a, b = on_u[i, 6], on_u[i, 7]
-        if a & 0x80000000:
-            return ("leaf", int(a & 0x7FFFFFFF), int(b))
-        return ("node", i)
-
-    def pref(r):
-        if r & 0x80000000:
-            return ("leaf", int(r & 0x0FFFFFFF), int((r >> 28) & 7))
-        return ("node", int(r))
-
-    stack = [(0, root)]
-    seen = 0
-    while stack:
-        oi, r = stack.pop()
-        o = oref(oi)
-        p = pref(r)
-        assert o[0] == p[0]
-        if o[0] == "leaf":
-            assert o[1:] == p[1:]
-            continue
-        seen += 1
-        n = nodes[p[1]]
-        nu = n.view(np.uint32)
-        la, lb = int(on_u[oi, 6]), int(on_u[oi, 7])
-        assert np.array_equal(n[0:3], onodes[la, 0:3]) and np.array_equal(n[4:7], onodes[la, 3:6])
-        assert np.array_equal(n[8:11], onodes[lb, 0:3]) and np.array_equal(n[12:15], onodes[lb, 3:6])
-        stack.append((la, int(nu[3])))
-        stack.append((lb, int(nu[7])))
-    assert seen == nodes.shape[0]
+    assert_tree_matches_oracle(prims, nodes, root, om)
 
 
 def test_host_postprocessing_matches_oracle(oracle):
@@ -153,11 +113,12 @@ def _host_build4(mb):
 
 @pytest.mark.parametrize("which", ["shoebox", "city", "soup"])
 def test_bvh4_collapse_covers_the_binary_tree(which):
-    """The 4-wide tree holds the binary tree's leaves once each, every leaf
-    child keeps the binary box of that leaf, every internal child's box is
-    the union of its node's children, 2-4 children per node in
-    breadth-first order, and the stack bound is the largest sum of
-    (children - 1) along a root-to-leaf path."""
+    """The 4-wide tree is the binary tree two levels at a time: node i of
+    binary node X holds in slots 0-1 the children of X's left child (or
+    that child itself in slot 0 when it is a leaf, slot 1 unused) and in
+    slots 2-3 those of its right child, with the binary boxes and leaf
+    references; 4-wide nodes are numbered breadth-first; the stack bound
+    is the largest sum of (children - 1) along a root-to-leaf path."""
     if which == "shoebox":
         mb = scenes.shoebox(seed=1)[0]
     elif which == "city":
@@ -173,50 +134,59 @@ def test_bvh4_collapse_covers_the_binary_tree(which):
     _, _, _, nodes, root, _ = _host_build(mb)
     n4, root4, stack4, usable = _host_build4(mb)
     nu = nodes.view(np.uint32)
-    # binary: leaf ref -> its box
-    leaf_box = {}
-    stack = [root]
-    while stack:
-        r = stack.pop()
-        if r & 0x80000000:
-            continue
-        for refc, lo, hi in ((nu[r, 3], 0, 4), (nu[r, 7], 8, 12)):
-            if refc & 0x80000000:
-                leaf_box[int(refc)] = (nodes[r, lo:lo + 3].copy(), nodes[r, hi:hi + 3].copy())
-            else:
-                stack.append(int(refc))
     if root & 0x80000000:
         assert root4 == root and n4.shape[0] == 0
         return
     assert root4 == 0
     u4 = n4.view(np.uint32)
-    seen = []
-    need = {}
+    NONE = 0xFFFFFFFF
 
-    def walk(i):
-        kids = [j for j in range(4) if u4[i, 8 * j + 3] != 0xFFFFFFFF]
-        assert 2 <= len(kids) <= 4 and kids == list(range(len(kids)))
-        deeper = 0
-        for j in kids:
-            ref = int(u4[i, 8 * j + 3])
-            mn, mx = n4[i, 8 * j:8 * j + 3], n4[i, 8 * j + 4:8 * j + 7]
+    def child(b, side):
+        """(ref, min, max) of binary node b's left (0) or right (1) child."""
+        o = 8 * side
+        return int(nu[b, 3 + 4 * side]), nodes[b, o:o + 3], nodes[b, o + 4:o + 7]
+
+    seen, need = [], {}
+    order = [(0, root)]          # breadth-first: (4-wide node, its binary node)
+    for i, b in order:
+        assert i == len(need)
+        kids = 0
+        for h in range(2):
+            ref, mn, mx = child(b, h)
             if ref & 0x80000000:
-                seen.append(ref)
-                bmn, bmx = leaf_box[ref]
-                assert np.array_equal(mn, bmn) and np.array_equal(mx, bmx)
+                want = [(ref, mn, mx), (NONE, None, None)]
             else:
-                assert ref > i
-                ck = [c for c in range(4) if u4[ref, 8 * c + 3] != 0xFFFFFFFF]
-                cmn = np.min([n4[ref, 8 * c:8 * c + 3] for c in ck], axis=0)
-                cmx = np.max([n4[ref, 8 * c + 4:8 * c + 7] for c in ck], axis=0)
-                assert np.array_equal(mn, cmn) and np.array_equal(mx, cmx)
-                deeper = max(deeper, walk(ref))
-        need[i] = len(kids) - 1 + deeper
-        return need[i]
-
-    assert walk(0) == stack4
-    assert sorted(seen) == sorted(leaf_box)
-    assert len(need) == n4.shape[0]
+                want = [child(ref, 0), child(ref, 1)]
+            for k, (wref, wmn, wmx) in enumerate(want):
+                j = 2 * h + k
+                got = int(u4[i, 8 * j + 3])
+                if wref == NONE:
+                    assert got == NONE
+                    continue
+                kids += 1
+                assert np.array_equal(n4[i, 8 * j:8 * j + 3], wmn)
+                assert np.array_equal(n4[i, 8 * j + 4:8 * j + 7], wmx)
+                if wref & 0x80000000:
+                    assert got == wref
+                    seen.append(wref)
+                else:
+                    assert got == len(order)
+                    order.append((got, wref))
+        need[i] = kids - 1
+    for i, _ in reversed(order):
+        deeper = [need[int(u4[i, 8 * j + 3])] for j in range(4)
+                  if not (u4[i, 8 * j + 3] & 0x80000000)]
+        need[i] += max(deeper, default=0)
+    assert need[0] == stack4
+    assert len(order) == n4.shape[0]
+    leaves = []
+    stack = [root]
+    while stack:
+        r = stack.pop()
+        for side in range(2):
+            ref = child(r, side)[0]
+            (leaves if ref & 0x80000000 else stack).append(ref)
+    assert sorted(seen) == sorted(leaves)
     assert usable == (1 if stack4 + 1 <= 64 else 0)
 
 

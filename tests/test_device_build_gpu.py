@@ -6,7 +6,9 @@ exact), the same tree summary; traces over either are bit-identical.
 Refit (omg_scene_refit) keeps the tree and recomputes every box: with
 unchanged vertices it reproduces the build, with moved vertices its hits
 equal the oracle's on the moved mesh except rays flagged as near-edge or
-exact ties, and it refuses moves that change the tessellation."""
+exact ties (the refit keeps the original tree, whose visiting order can
+differ from a fresh build's among hits at equal distance), and it
+refuses moves that change the tessellation."""
 import sys
 
 import numpy as np
@@ -15,7 +17,7 @@ import pytest
 import oracle as O
 import omg_hip as H
 from omg_hip import scenes
-from helpers import oracle_mesh_from_builder
+from helpers import assert_tree_matches_oracle, oracle_mesh_from_builder
 
 pytestmark = pytest.mark.gpu
 
@@ -91,7 +93,7 @@ def _case(name):
 
 @pytest.mark.parametrize("name", ["shoebox", "city", "soup_tess", "coarse", "stacked", "tiny1",
                                   "tiny4", "tiny5", "tiny9"])
-def test_device_build_equals_host_build(ctx, name):
+def test_device_build_equals_host_build(ctx, oracle, name):
     mb = _case(name)
     m = _mesh(mb)
     _say(f"case device build {name}: {len(m['indices'])} tris")
@@ -99,8 +101,11 @@ def test_device_build_equals_host_build(ctx, name):
     try:
         a, b = host.export(), dev.export()
         _same_tree(a, b)
+        # and the device tree against the oracle's own build, node for node
+        pb, nb, _, tb = b
+        assert_tree_matches_oracle(pb, nb, tb["root_ref"], oracle_mesh_from_builder(mb))
         _say(f"  {a[3]['n_prims']} prims, {a[3]['n_nodes']} nodes, {a[3]['n_nodes4']} 4-wide, "
-             f"depth {a[3]['max_depth']}: identical")
+             f"depth {a[3]['max_depth']}: identical to the host build and the oracle")
     finally:
         host.close()
         dev.close()
