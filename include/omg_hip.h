@@ -128,6 +128,15 @@ int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds,
  * to the order of their f32 sums.  Default 4194304; 0 = never.  No
  * reference counterpart (the reference has one kernel per call). */
 int omg_set_route_rays(omg_ctx *ctx, uint64_t min_rays);
+/* Plain launches that trace a quad per ray over the 4-wide tree (below the
+ * routing threshold, with the whole stack in LDS) hand every ray still
+ * alive after `bounce` bounces to a second kernel that continues it on a
+ * 16-lane row over the 16-wide nodes: the first bounces keep all rays in
+ * flight four lanes each, the long-lived rays then walk four binary levels
+ * per node fetch.  The same rays and per-ray records; echograms equal up
+ * to the order of their f32 sums.  Default 8; 0 = off.  No reference
+ * counterpart. */
+int omg_set_handoff_bounce(omg_ctx *ctx, uint32_t bounce);
 
 /* ------------------------------------------------------------ scenes */
 
@@ -193,6 +202,9 @@ typedef struct {
     uint32_t n_prims, n_nodes, n_nodes4;
     uint32_t root_ref, root4, stack4, has4, max_depth, empty, pad;
     float root_min[3], root_max[3];
+    /* the 16-wide nodes derived from the 4-wide ones (16-lane walks): their
+     * count, worst-case stack and whether the stack fits (has16) */
+    uint32_t n_nodes16, stack16, has16, pad16;
 } omg_scene_tree;
 
 /* The resident tree of any scene, in the row layouts of omg_bvh_build_host

@@ -49,6 +49,7 @@ struct omg_batch {
     BatchShape shape{1, 16, 4, 1, 2};
     bool all4 = true;              // every scene has its 4-wide tree
     bool deep = true;              // ... and its stack fits kDeepStack entries
+    bool all16 = true;             // every scene has a usable 16-wide tree
     float shape_ms = 0.0f;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;
@@ -93,6 +94,10 @@ int capture(omg_batch* b)
     if (b->flags & OMG_BATCH_COUNTERS) {
         sh.width = 2;
         if (sh.stack_lds == kDeepStack) sh.stack_lds = 16;
+        if (sh.lanes == 16) {
+            sh.lanes = 4;
+            sh.stack_lds = 16;
+        }
     }
     if (e == hipSuccess)
         e = launch_trace_batch(B, b->nbm, sh, c->n_cu, (b->flags & OMG_BATCH_COUNTERS) != 0,
@@ -137,7 +142,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
     uint32_t nbm = 1;
     uint64_t rays = 0, items = 0;
     double lg_prims = 0.0;
-    bool all4 = true, deep = true;
+    bool all4 = true, deep = true, all16 = true;
     for (uint32_t i = 0; i < n; i++) {
         const omg_query& Q = queries[i];
         if (!Q.scene || Q.scene->ctx != c || !Q.sources || !Q.listeners || (Q.n_ovl && !Q.overlays))
@@ -146,6 +151,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         if (rc) return fail(rc, "query " + std::to_string(i) + ": " + omg_last_error());
         nbm = std::max(nbm, Q.scene->nbands);
         all4 = all4 && Q.scene->has4;
+        all16 = all16 && Q.scene->has16;
         deep = deep && Q.scene->has4 && Q.scene->stack4 <= kDeepStack;
         uint64_t r = (uint64_t)Q.params.n_rays * Q.n_src;
         rays += r;
@@ -161,6 +167,7 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
     b->nbm = nbm;
     b->rays = rays;
     b->all4 = all4;
+    b->all16 = all16;
     b->deep = deep;
     b->q.resize(n);
     // host staging of every query's tables: mats | overlays | sources | listeners
@@ -274,7 +281,8 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         // untuned default: the regenerating walk (4-wide where every scene has it)
         b->shape = all4 ? BatchShape{1, 12, 6, 1, 4} : BatchShape{1, 8, 8, 1, 2};
         auto it = c->tuned.find(b->class_key);
-        if (it != c->tuned.end() && (it->second.width == 2 || all4) &&
+        if (it != c->tuned.end() && (it->second.width == 2 || (it->second.width == 4 && all4) ||
+                                     (it->second.width == 16 && all16)) &&
             (it->second.stack_lds != kDeepStack || deep))
             b->shape = it->second;
         rc = capture(b);
@@ -380,8 +388,11 @@ extern "C" int omg_batch_set_shape(omg_batch* b, const omg_batch_shape* s)
         return fail(OMG_ERR_INVALID,
                     "shape: lanes 1|4 with stack_lds 16|32 (width 2) or 16 (width 4); regen 1 "
                     "with lanes 1, stack_lds 8|12|16 and width 2|4, or lanes 4, stack_lds 16 and "
-                    "width 2; lanes 4 with stack_lds 40 and width 4; waves_per_simd 1..8");
+                    "width 2; lanes 4 with stack_lds 40 and width 4; regen 1 with lanes 16, "
+                    "stack_lds 96 and width 16; waves_per_simd 1..8");
     if (sh.width == 4 && !b->all4) return fail(OMG_ERR_INVALID, "a scene of the batch has no 4-wide tree");
+    if (sh.width == 16 && !b->all16)
+        return fail(OMG_ERR_INVALID, "a scene of the batch has no usable 16-wide tree");
     if (sh.stack_lds == kDeepStack && !b->deep)
         return fail(OMG_ERR_INVALID, "a scene of the batch needs more than 40 stack entries");
     OMG_HIP_CHECK(hipSetDevice(b->ctx->device));
@@ -441,14 +452,16 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
     // (lanes, stack_lds, regen, width): a lane or a quad per ray, with and
     // without regeneration, over the binary or the 4-wide tree; each at
     // 2..8 resident waves per SIMD
-    const uint32_t modes[15][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
+    const uint32_t modes[16][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
                                    {1u, 8u, 1u, 4u},  {1u, 12u, 1u, 4u}, {1u, 16u, 1u, 4u},
                                    {4u, 16u, 1u, 2u}, {4u, 40u, 1u, 4u}, {1u, 16u, 0u, 2u},
                                    {1u, 32u, 0u, 2u}, {4u, 16u, 0u, 2u}, {4u, 32u, 0u, 2u},
-                                   {1u, 16u, 0u, 4u}, {4u, 16u, 0u, 4u}, {4u, 40u, 0u, 4u}};
-    static_assert(kDeepStack == 40, "tuner modes name the deep stack");
+                                   {1u, 16u, 0u, 4u}, {4u, 16u, 0u, 4u}, {4u, 40u, 0u, 4u},
+                                   {16u, 96u, 1u, 16u}};
+    static_assert(kDeepStack == 40 && kStack16 == 96, "tuner modes name the deep stacks");
     for (const auto& m : modes) {
         if (m[3] == 4 && !b->all4) continue;
+        if (m[3] == 16 && !b->all16) continue;
         if (m[1] == kDeepStack && !b->deep) continue;
         for (uint32_t wps : {2u, 4u, 6u, 8u}) {
             BatchShape sh{m[0], m[1], wps, m[2], m[3]};

@@ -129,8 +129,8 @@ int alloc_collapse(Scratch& X, size_t base)
 void scene_free(omg_scene* s)
 {
     for (DevBuf* b : {&s->nodes, &s->prims, &s->tris, &s->mats, &s->mats_raw, &s->tri_surf,
-                      &s->nodes4, &s->b_pos, &s->b_idx, &s->b_mat, &s->b_surf, &s->b_maxe,
-                      &s->b_cnt, &s->b_off, &s->leaf_of, &s->src4})
+                      &s->nodes4, &s->nodes16, &s->b_pos, &s->b_idx, &s->b_mat, &s->b_surf,
+                      &s->b_maxe, &s->b_cnt, &s->b_off, &s->leaf_of, &s->src4})
         b->release();
     delete s;
 }
@@ -472,6 +472,7 @@ extern "C" int omg_scene_create_device(omg_ctx* c, const float* positions, uint3
         return build_tree(s, X, n, st);
     };
     int rc = run();
+    if (!rc) rc = derive16(s, st);
     if (!rc) rc = upload(s->mats, s->mat_params.data(), s->mat_params.size() * sizeof(MatParams), st);
     if (!rc) rc = upload(s->mats_raw, s->materials.data(), s->materials.size() * sizeof(omg_material), st);
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "device scene build");
@@ -482,8 +483,8 @@ extern "C" int omg_scene_create_device(omg_ctx* c, const float* positions, uint3
     }
     s->device_bytes = 0;
     for (DevBuf* b : {&s->nodes, &s->prims, &s->tris, &s->mats, &s->mats_raw, &s->tri_surf,
-                      &s->nodes4, &s->b_pos, &s->b_idx, &s->b_mat, &s->b_surf, &s->b_maxe,
-                      &s->b_cnt, &s->b_off, &s->leaf_of, &s->src4})
+                      &s->nodes4, &s->nodes16, &s->b_pos, &s->b_idx, &s->b_mat, &s->b_surf,
+                      &s->b_maxe, &s->b_cnt, &s->b_off, &s->leaf_of, &s->src4})
         s->device_bytes += b->cap;
     c->live_scenes++;
     *out = s;
@@ -509,6 +510,9 @@ extern "C" int omg_scene_export(omg_scene* s, float* prims_out, uint32_t prim_ca
         tree->root_min[k] = s->root_min[k];
         tree->root_max[k] = s->root_max[k];
     }
+    tree->n_nodes16 = s->n_nodes16;
+    tree->stack16 = s->stack16;
+    tree->has16 = s->has16 ? 1u : 0u;
     hipStream_t st = s->ctx->stream;
     OMG_HIP_CHECK(hipSetDevice(s->ctx->device));
     if (int rc = quiesce(s->ctx)) return rc;
@@ -583,6 +587,7 @@ extern "C" int omg_scene_refit(omg_scene* s, const float* positions, uint32_t nv
     if (s->has4 && s->n_nodes4)
         OMG_HIP_CHECK(launch_refit4(s->nodes.as<Node2>(), s->src4.as<uint32_t>(), s->nodes4.as<Node4>(),
                                     s->n_nodes4, st));
+    if (int rc = derive16(s, st)) return rc;
     BuildMeta meta{};
     if (int rc = d2h(&meta, X.meta.ptr, sizeof meta, st)) return rc;
     for (int k = 0; k < 3; k++) {
@@ -590,4 +595,99 @@ extern "C" int omg_scene_refit(omg_scene* s, const float* positions, uint32_t nv
         s->root_max[k] = meta.root_max[k];
     }
     return OMG_OK;
+}
+
+// The 16-wide nodes of a scene, derived from its resident 4-wide nodes
+// level by level (the 4-wide collapse's breadth-first scheme: expand, scan
+// of the internal children, link).  A 16-wide node takes a 4-wide node and
+// expands each of its internal slots once more, so its internal children
+// are the 4-wide nodes two levels down; their count per level is read back
+// to size the next level.  Stream-ordered; the working buffers are freed
+// before returning.
+int omg::derive16(omg_scene* s, hipStream_t st)
+{
+    s->nodes16.release();
+    s->n_nodes16 = 0;
+    s->stack16 = 0;
+    s->has16 = false;
+    if (!s->has4) return OMG_OK;
+    if (s->empty || (s->root4 & kLeafBit) || s->n_nodes4 == 0) {
+        // no internal node: the walk tests the root leaf (or nothing)
+        s->root16 = s->root4;
+        s->has16 = true;
+        return OMG_OK;
+    }
+    const size_t n4 = s->n_nodes4;
+    DevBuf out, work;
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    auto done = [&](int rc) {
+        (void)hipStreamSynchronize(st);
+        out.release();
+        work.release();
+        if (scan_tmp) (void)hipFree(scan_tmp);
+        return rc;
+    };
+    uint32_t *queue[2], *path[2], *nint, *iscan, *stack16;
+    Carve M;
+    for (int k = 0; k < 2; k++) {
+        queue[k] = M.take<uint32_t>(n4 + 1);
+        path[k] = M.take<uint32_t>(n4 + 1);
+    }
+    nint = M.take<uint32_t>(n4 + 2);
+    iscan = M.take<uint32_t>(n4 + 2);
+    stack16 = M.take<uint32_t>(1);
+    if (out.ensure_exact(n4 * sizeof(Node16)) != hipSuccess || work.ensure_exact(M.off) != hipSuccess)
+        return done(fail(OMG_ERR_DEVICE, "16-wide node memory"));
+    Carve C;
+    C.base = work.as<char>();
+    for (int k = 0; k < 2; k++) {
+        queue[k] = C.take<uint32_t>(n4 + 1);
+        path[k] = C.take<uint32_t>(n4 + 1);
+    }
+    nint = C.take<uint32_t>(n4 + 2);
+    iscan = C.take<uint32_t>(n4 + 2);
+    stack16 = C.take<uint32_t>(1);
+    const uint32_t zero = 0;
+    if (hipMemcpyAsync(queue[0], &zero, 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(path[0], &zero, 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(stack16, &zero, 4, hipMemcpyHostToDevice, st) != hipSuccess)
+        return done(fail(OMG_ERR_DEVICE, "16-wide derivation"));
+    uint32_t q = 1, n16 = 0;
+    int qc = 0;
+    while (q) {
+        Derive16Args D{};
+        D.nodes4 = s->nodes4.as<Node4>();
+        D.nodes16 = out.as<Node16>();
+        D.q = q;
+        D.base16 = n16;
+        D.base16_next = n16 + q;
+        D.queue = queue[qc];
+        D.queue_next = queue[qc ^ 1];
+        D.path = path[qc];
+        D.path_next = path[qc ^ 1];
+        D.n_int = nint;
+        D.int_scan = iscan;
+        D.stack16 = stack16;
+        if (launch_derive16_expand(D, st) != hipSuccess ||
+            scan_u32(scan_tmp, scan_bytes, nint, iscan, (size_t)q + 1, st) != hipSuccess ||
+            launch_derive16_link(D, st) != hipSuccess)
+            return done(fail(OMG_ERR_DEVICE, "16-wide derivation"));
+        uint32_t next = 0;
+        if (int rc = d2h(&next, iscan + q, 4, st)) return done(rc);
+        n16 += q;
+        q = next;
+        qc ^= 1;
+    }
+    uint32_t sk = 0;
+    if (int rc = d2h(&sk, stack16, 4, st)) return done(rc);
+    if (s->nodes16.ensure_exact((size_t)n16 * sizeof(Node16)) != hipSuccess ||
+        hipMemcpyAsync(s->nodes16.ptr, out.ptr, (size_t)n16 * sizeof(Node16), hipMemcpyDeviceToDevice,
+                       st) != hipSuccess)
+        return done(fail(OMG_ERR_DEVICE, "16-wide node memory"));
+    s->n_nodes16 = n16;
+    s->root16 = 0;
+    s->stack16 = sk;
+    s->has16 = sk <= kStack16;
+    return done(OMG_OK);
 }

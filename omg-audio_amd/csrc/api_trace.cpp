@@ -56,10 +56,13 @@ static void ctx_teardown(omg_ctx* c)
 {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    // async launches on the caller's streams may still read the context's buffers
+    if (c->async_inflight) (void)hipDeviceSynchronize();
     for (DevBuf* b : {&c->job_mats, &c->job_ovl, &c->job_src, &c->job_lis, &c->job_bins,
                       &c->job_dirs, &c->job_rec, &c->job_cnt, &c->aux0, &c->aux1, &c->aux2,
                       &c->aux3, &c->aux4, &c->fan_dirs, &c->fan_bins, &c->dome_status,
-                      &c->dome_panels, &c->route_job, &c->route_items, &c->route_cursor})
+                      &c->dome_panels, &c->route_job, &c->route_items, &c->route_cursor,
+                      &c->handoff, &c->handoff_ctl})
         b->release();
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -78,9 +81,10 @@ extern "C" int omg_set_launch_config(omg_ctx* c, uint32_t node_cache, uint32_t s
                                      uint32_t lanes, uint32_t width)
 {
     if (!c) return fail(OMG_ERR_INVALID, "bad argument");
-    if (width != 2 && width != 4) return fail(OMG_ERR_INVALID, "width must be 2 or 4");
+    if (width != 2 && width != 4 && width != 16) return fail(OMG_ERR_INVALID, "width must be 2, 4 or 16");
     if (stack_lds != 16 && stack_lds != 32) return fail(OMG_ERR_INVALID, "stack_lds must be 16 or 32");
-    if (lanes != 0 && lanes != 1 && lanes != 4) return fail(OMG_ERR_INVALID, "lanes must be 0, 1 or 4");
+    if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 16)
+        return fail(OMG_ERR_INVALID, "lanes must be 0, 1, 4 or 16");
     size_t lds = (size_t)node_cache * sizeof(Node2) + (size_t)stack_lds * 256 * 8;
     if (lds > 160 * 1024) return fail(OMG_ERR_CAP, "LDS working set above 160 KiB");
     c->node_cache = node_cache;
@@ -97,12 +101,83 @@ extern "C" int omg_set_route_rays(omg_ctx* c, uint64_t min_rays)
     return OMG_OK;
 }
 
+extern "C" int omg_set_handoff_bounce(omg_ctx* c, uint32_t bounce)
+{
+    if (!c) return fail(OMG_ERR_INVALID, "bad argument");
+    c->handoff_bounce = bounce;
+    return OMG_OK;
+}
+
 // Lanes per ray for a launch of n rays: a quad per ray while one lane per
 // ray would leave the 1024 SIMDs with few waves to switch between.
 uint32_t omg::pick_lanes(const omg_ctx* c, uint64_t n_rays)
 {
     if (c->lanes) return c->lanes;
     return n_rays < c->quad_below ? 4u : 1u;
+}
+
+void omg::plain_shape(const omg_ctx* c, const omg_scene* s, const omg_trace_params* p,
+                      uint32_t n_src, bool counting, TraceArgs& A)
+{
+    const uint64_t rays = (uint64_t)p->n_rays * n_src;
+    A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
+    A.stack_lds = c->stack_lds;
+    A.lanes = pick_lanes(c, rays);
+    A.width = 2;
+    if (counting) {
+        // traversal counters are the reference walk's: counted on the binary tree
+        if (A.lanes == 16) A.lanes = 4;
+        return;
+    }
+    if (A.lanes == 16 || c->width == 16) {
+        if (s->has16 && s->n_nodes16) {
+            // a 16-lane row per ray over the 16-wide nodes, the whole stack in LDS
+            A.lanes = 16;
+            A.width = 16;
+            A.stack_lds = kStack16;
+            A.node_cache = std::min(s->n_nodes16, c->node_cache / 8);
+            return;
+        }
+        A.lanes = 4;
+    }
+    if (c->width >= 4 && s->has4) {
+        A.width = 4;
+        A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
+        if (A.lanes == 4 && s->stack4 <= kDeepStack && c->stack_lds == 16) {
+            // the quad walk keeps its whole stack in LDS: 20 KB per block, so
+            // the node prefix shrinks to keep a block within 40 KB
+            A.stack_lds = kDeepStack;
+            A.node_cache = std::min(A.node_cache, 160u);
+            if (c->handoff_bounce && s->has16 && s->n_nodes16 && p->max_bounces > c->handoff_bounce &&
+                !(c->route_rays && rays >= c->route_rays)) {
+                // the rays still alive after the first bounces continue on a
+                // 16-lane row each (see k_trace_resume16)
+                A.handoff_bounce = c->handoff_bounce;
+                A.resume_blocks = c->n_cu * 6u;
+                A.node_cache16 = std::min(s->n_nodes16, 16u);
+            }
+        }
+    }
+}
+
+int omg::setup_handoff(omg_ctx* c, TraceArgs& A, hipStream_t st)
+{
+    if (!A.handoff_bounce) return OMG_OK;
+    const size_t bytes = (size_t)A.ray_count * kHandoffF4 * sizeof(float4);
+    // the buffers are the context's: a launch on another stream, or one that
+    // regrows them, waits for the launches that may still use them
+    if ((bytes > c->handoff.cap || st != c->handoff_stream) && c->handoff_stream) {
+        int rc = quiesce(c);
+        if (rc) return rc;
+        OMG_HIP_CHECK(hipStreamSynchronize(c->handoff_stream));
+    }
+    OMG_HIP_CHECK(c->handoff.ensure(bytes));
+    OMG_HIP_CHECK(c->handoff_ctl.ensure(2 * sizeof(uint32_t)));
+    c->handoff_stream = st;
+    A.handoff = c->handoff.as<float4>();
+    A.handoff_ctl = c->handoff_ctl.as<uint32_t>();
+    OMG_HIP_CHECK(launch_fill_zero(reinterpret_cast<float*>(A.handoff_ctl), 2, st));
+    return OMG_OK;
 }
 
 extern "C" int omg_device_name(omg_ctx* c, char* buf, uint32_t len)
@@ -177,6 +252,7 @@ extern "C" int omg_scene_create(omg_ctx* c, const float* positions, uint32_t nv,
     std::vector<uint32_t> ts32(s->tri_surface.begin(), s->tri_surface.end());
     if (!rc) rc = upload(s->tri_surf, ts32.data(), ts32.size() * sizeof(uint32_t), c->stream);
     if (!rc && s->has4) rc = upload(s->nodes4, H.nodes4.data(), H.nodes4.size() * sizeof(Node4), c->stream);
+    if (!rc) rc = derive16(s, c->stream);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "scene upload");
     if (rc) {
         s->nodes.release();
@@ -186,11 +262,12 @@ extern "C" int omg_scene_create(omg_ctx* c, const float* positions, uint32_t nv,
         s->mats_raw.release();
         s->tri_surf.release();
         s->nodes4.release();
+        s->nodes16.release();
         delete s;
         return rc;
     }
     s->device_bytes = s->nodes.cap + s->prims.cap + s->tris.cap + s->mats.cap + s->mats_raw.cap +
-                      s->tri_surf.cap + s->nodes4.cap;
+                      s->tri_surf.cap + s->nodes4.cap + s->nodes16.cap;
     c->live_scenes++;
     *out = s;
     return OMG_OK;
@@ -298,6 +375,7 @@ extern "C" void omg_scene_destroy(omg_scene* s)
     s->mats_raw.release();
     s->tri_surf.release();
     s->nodes4.release();
+    s->nodes16.release();
     for (DevBuf* b : {&s->b_pos, &s->b_idx, &s->b_mat, &s->b_surf, &s->b_maxe, &s->b_cnt,
                       &s->b_off, &s->leaf_of, &s->src4})
         b->release();
@@ -482,6 +560,8 @@ int run_trace_host(omg_ctx* c, TraceArgs& A, bool mesh, uint32_t nbins, uint32_t
         rc = launch_routed(c, A, nbands, counters != nullptr, c->stream);
         if (rc) return rc;
     } else {
+        rc = setup_handoff(c, A, c->stream);
+        if (rc) return rc;
         OMG_HIP_CHECK(launch_trace(A, mesh, counters != nullptr, c->stream));
     }
     OMG_HIP_CHECK(hipMemcpyAsync(bins, A.bins, nb_f * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -513,8 +593,10 @@ void omg::fill_geom(const omg_scene* s, GeomArgs& g)
     g.prims = s->prims.as<Prim>();
     g.tris = s->tris.as<TriInfo>();
     g.nodes4 = s->has4 ? s->nodes4.as<Node4>() : nullptr;
+    g.nodes16 = s->has16 ? s->nodes16.as<Node16>() : nullptr;
     g.root_ref = s->root_ref;
     g.root4 = s->root4;
+    g.root16 = s->root16;
     g.empty = s->empty ? 1u : 0u;
     g.box[0] = g.box[1] = g.box[2] = 0.0f;
     for (int k = 0; k < 3; k++) {
@@ -537,18 +619,7 @@ extern "C" int omg_trace_world(omg_scene* s, const omg_trace_params* p, const fl
     OMG_HIP_CHECK(hipSetDevice(c->device));
     TraceArgs A{};
     fill_geom(s, A.g);
-    A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
-    A.stack_lds = c->stack_lds;
-    A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
-    // traversal counters are the reference walk's: counted on the binary tree
-    A.width = c->width == 4 && s->has4 && !counters ? 4u : 2u;
-    if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
-    if (A.width == 4 && A.lanes == 4 && s->stack4 <= kDeepStack && c->stack_lds == 16) {
-        // the quad walk keeps its whole stack in LDS: 20 KB per block, so
-        // the node prefix shrinks to keep a block within 40 KB
-        A.stack_lds = kDeepStack;
-        A.node_cache = std::min(A.node_cache, 160u);
-    }
+    plain_shape(c, s, p, n_src, counters != nullptr, A);
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
@@ -573,17 +644,7 @@ extern "C" int omg_trace_world_hits(omg_scene* s, const omg_trace_params* p, con
     OMG_HIP_CHECK(hipSetDevice(c->device));
     TraceArgs A{};
     fill_geom(s, A.g);
-    A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
-    A.stack_lds = c->stack_lds;
-    A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
-    A.width = c->width == 4 && s->has4 ? 4u : 2u;
-    if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
-    if (A.width == 4 && A.lanes == 4 && s->stack4 <= kDeepStack && c->stack_lds == 16) {
-        // the quad walk keeps its whole stack in LDS: 20 KB per block, so
-        // the node prefix shrinks to keep a block within 40 KB
-        A.stack_lds = kDeepStack;
-        A.node_cache = std::min(A.node_cache, 160u);
-    }
+    plain_shape(c, s, p, n_src, false, A);
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
     return run_trace_host(c, A, true, nbins, s->nbands, sources, n_src, listeners, n_lis, bins,
@@ -634,17 +695,7 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     TraceArgs A{};
     fill_geom(s, A.g);
-    A.node_cache = s->n_nodes < c->node_cache ? s->n_nodes : c->node_cache;
-    A.stack_lds = c->stack_lds;
-    A.lanes = pick_lanes(c, (uint64_t)p->n_rays * n_src);
-    A.width = c->width == 4 && s->has4 ? 4u : 2u;
-    if (A.width == 4) A.node_cache = std::min(s->n_nodes4, c->node_cache / 2);
-    if (A.width == 4 && A.lanes == 4 && s->stack4 <= kDeepStack && c->stack_lds == 16) {
-        // the quad walk keeps its whole stack in LDS: 20 KB per block, so
-        // the node prefix shrinks to keep a block within 40 KB
-        A.stack_lds = kDeepStack;
-        A.node_cache = std::min(A.node_cache, 160u);
-    }
+    plain_shape(c, s, p, n_src, false, A);
     // the job tables are staged on the context stream, then ordered before the launch
     rc = prepare_job(c, p, s->mat_params, overlays, n_ovl, s->nbands, A);
     if (rc) return rc;
@@ -670,6 +721,8 @@ extern "C" int omg_trace_world_async(omg_scene* s, const omg_trace_params* p,
         c->async_inflight = true;
         return rc;
     }
+    rc = setup_handoff(c, A, st);
+    if (rc) return rc;
     OMG_HIP_CHECK(launch_fill_zero(d_bins, nb_f, st));
     OMG_HIP_CHECK(launch_fill_zero(d_dirs, nd_f, st));
     c->async_inflight = true;

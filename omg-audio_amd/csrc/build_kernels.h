@@ -141,6 +141,23 @@ struct Collapse4Args {
     BuildMeta* meta;
 };
 
+// One level of the 16-wide derivation (Node16 from the 4-wide nodes).
+struct Derive16Args {
+    const Node4* nodes4;
+    Node16* nodes16;
+    uint32_t q, base16, base16_next;
+    const uint32_t* queue;        // 4-wide node of each 16-wide node at this level
+    uint32_t* queue_next;
+    const uint32_t* path;         // stack entries above this level's nodes
+    uint32_t* path_next;
+    uint32_t* n_int;              // [q + 1] internal children per node
+    const uint32_t* int_scan;     // exclusive scan of n_int
+    uint32_t* stack16;            // worst-case stack (atomic max)
+};
+
+hipError_t launch_derive16_expand(const Derive16Args& A, hipStream_t s);
+hipError_t launch_derive16_link(const Derive16Args& A, hipStream_t s);
+
 hipError_t launch_tess_count(const TessArgs& A, hipStream_t s);
 hipError_t launch_tess_seed(const TessArgs& A, TessItem* items, hipStream_t s);
 hipError_t launch_tess_round_count(const TessRound& R, hipStream_t s);

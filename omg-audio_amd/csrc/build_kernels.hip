@@ -930,6 +930,70 @@ __global__ void k_collapse_link(Collapse4Args A)
     atomicMax(&A.meta->stack4, here);
 }
 
+// One 16-wide node from its 4-wide node: slot group 4k..4k+3 is the
+// 4-wide node's slot k expanded once more (the four slots of the 4-wide
+// node it refers to; a leaf in slot 4k; or four unused slots).  Internal
+// children keep their 4-wide reference until k_derive16_link.
+__global__ void k_derive16_expand(Derive16Args A)
+{
+    uint32_t j = gid();
+    if (j == A.q) A.n_int[A.q] = 0;
+    if (j >= A.q) return;
+    const Node4 a = A.nodes4[A.queue[j]];
+    Node16 o;
+    uint32_t ni = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const Child4 c = a.c[k];
+        Child4 none;
+        none.mn[0] = none.mn[1] = none.mn[2] = 0.0f;
+        none.mx[0] = none.mx[1] = none.mx[2] = 0.0f;
+        none.ref = kNoChild;
+        none.pad = 0;
+        if (c.ref == kNoChild || (c.ref & kLeafBit)) {
+            o.c[4 * k] = c;
+            o.c[4 * k].pad = 0;
+            o.c[4 * k + 1] = none;
+            o.c[4 * k + 2] = none;
+            o.c[4 * k + 3] = none;
+        } else {
+            const Node4 m = A.nodes4[c.ref];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                o.c[4 * k + i] = m.c[i];
+                o.c[4 * k + i].pad = 0;
+                const uint32_t r = m.c[i].ref;
+                if (r != kNoChild && !(r & kLeafBit)) ni++;
+            }
+        }
+    }
+    A.nodes16[A.base16 + j] = o;
+    A.n_int[j] = ni;
+}
+
+// Breadth-first numbering of the next level's 16-wide nodes and the
+// worst-case stack: the largest sum of (children - 1) along a path.
+__global__ void k_derive16_link(Derive16Args A)
+{
+    uint32_t j = gid();
+    if (j >= A.q) return;
+    Node16* o = A.nodes16 + A.base16 + j;
+    uint32_t off = A.int_scan[j];
+    uint32_t nk = 0;
+    for (int t = 0; t < 16; t++)
+        if (o->c[t].ref != kNoChild) nk++;
+    uint32_t here = A.path[j] + nk - 1;
+    for (int t = 0; t < 16; t++) {
+        uint32_t ref = o->c[t].ref;
+        if (ref == kNoChild || (ref & kLeafBit)) continue;
+        A.queue_next[off] = ref;
+        A.path_next[off] = here;
+        o->c[t].ref = A.base16_next + off;
+        off++;
+    }
+    atomicMax(A.stack16, here);
+}
+
 __device__ __forceinline__ void leaf_box(const Prim* prims, uint32_t ref, f3& lo, f3& hi)
 {
     uint32_t start = ref & kLeafStartMask, cnt = (ref >> kLeafCountShift) & 7u;
@@ -997,6 +1061,19 @@ __global__ void k_refit4(const Node2* nodes, const uint32_t* src4, Node4* nodes4
 }
 
 }  // namespace
+
+hipError_t launch_derive16_expand(const Derive16Args& A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_derive16_expand, grid_for((size_t)A.q + 1), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_derive16_link(const Derive16Args& A, hipStream_t s)
+{
+    if (!A.q) return hipSuccess;
+    hipLaunchKernelGGL(k_derive16_link, grid_for(A.q), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
 
 hipError_t launch_tess_count(const TessArgs& A, hipStream_t s)
 {

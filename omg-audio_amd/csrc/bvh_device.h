@@ -333,11 +333,14 @@ __device__ __forceinline__ bool mesh_raycast_lds(const GeomArgs& G, const LdsTra
 // with DPP quad_perm moves and folded in the reference's order (left/right,
 // then primitives in leaf order), so the outcome is the one-lane traversal's.
 
+// Broadcast of lane Q of each quad.  Every lane of a quad is active where
+// it is used (the quad shares its control flow), so bound_ctrl may be set,
+// which lets the compiler fold the move into the instruction using it.
 template <int Q>
 __device__ __forceinline__ uint32_t qb(uint32_t x)
 {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, Q | (Q << 2) | (Q << 4) | (Q << 6), 0xF, 0xF,
-                                              false);
+                                              true);
 }
 template <int Q>
 __device__ __forceinline__ float qbf(float x) { return __uint_as_float(qb<Q>(__float_as_uint(x))); }
@@ -464,7 +467,7 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
 // entries), and inside a half the nearer slot first (again left on ties).
 // The halves' boxes are not stored: a half's box is the union of its
 // slots' boxes, whose slab entry follows from the slots' per-axis entries
-// (half_entry).  Culling is the binary walk's (`t_enter >= best` at pop,
+// (entry_key, half_key).  Culling is the binary walk's (`t_enter >= best` at pop,
 // the current best as tmax of the box test), so the 4-wide walk visits the
 // binary walk's leaves in the binary walk's order with the same best
 // distance at every step, and its nearest hit - triangle, distance and the
@@ -510,27 +513,27 @@ __device__ __forceinline__ bool slab_entries(const float4& lo, const float4& hi,
     return !(t0 > t1);
 }
 
-// Least of two per-axis entries as an entry of the union box.  With a
-// finite 1/d every entry is a number (an overflow gives an infinity) and
-// rounding is monotone, so the union's entry is the least of its parts'.
-// With an infinite 1/d (a zero direction component) an entry is NaN when
-// the origin lies on that box plane; the slab test's fmaxf then ignores
-// it, which is what -inf does, and the union's entry is NaN (ignored)
-// exactly when one of its parts' is.
-__device__ __forceinline__ float union_entry(float a, float b, bool infax)
-{
-    float m = fminf(a, b);
-    if (infax) m = (a != a || b != b) ? -__builtin_inff() : m;
-    return m;
-}
+// Entry keys.  A slot's per-axis entry e enters the group computations as
+// the key max(0, e) in the bits of the float (a NaN or -0 gives 0):
+// non-negative floats and +inf order as their bit patterns, so the entry
+// of a union box is the integer minimum of its parts' keys per axis and
+// a box's entry distance the integer maximum over its axes.  This holds
+// for a finite 1/d, where every entry is a number (an overflow an
+// infinity) and rounding is monotone: the union's per-axis entry is the
+// least of its parts', and max(0, .) commutes with the minimum.  A ray
+// with an infinite 1/d component (a zero direction component) can meet
+// NaN entries, for which neither holds; the wide walks hand such rays to
+// the binary walk (mesh_raycast), which is the reference walk itself.
+__device__ __forceinline__ uint32_t entry_key(float e) { return e > 0.0f ? __float_as_uint(e) : 0u; }
 
-// Entry distance of a half's box from its two slots' per-axis entries.
-__device__ __forceinline__ float half_entry(const float ea[3], const float eb[3], bool infax)
+// Entry key of the union of two slots from their per-axis keys.
+__device__ __forceinline__ uint32_t half_key(const uint32_t ka[3], const uint32_t kb[3])
 {
-    float t = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 3; a++) t = fmaxf(t, union_entry(ea[a], eb[a], infax));
-    return t;
+    const uint32_t x = ka[0] < kb[0] ? ka[0] : kb[0];
+    const uint32_t y = ka[1] < kb[1] ? ka[1] : kb[1];
+    const uint32_t z = ka[2] < kb[2] ? ka[2] : kb[2];
+    const uint32_t m = x > y ? x : y;
+    return m > z ? m : z;
 }
 
 // The binary walk's visiting order of a node's four slots (p[0] first):
@@ -547,37 +550,33 @@ __device__ __forceinline__ void binary_order(const Cand4 c[4], float tl, float t
     p[3] = csel(rx, lf, rf);
 }
 
-// Position of slot j in binary_order's sequence.
-__device__ __forceinline__ uint32_t binary_key(uint32_t j, bool rx, bool r0, bool r1)
-{
-    const uint32_t far_half = (j >> 1) ^ (rx ? 1u : 0u);
-    const uint32_t far_slot = (j & 1u) ^ ((j < 2 ? r0 : r1) ? 1u : 0u);
-    return (far_half << 1) | far_slot;
-}
-
 // Tests the four slots of a node in registers (slot j: box lo[j].xyz ..
-// hi[j].xyz, ref lo[j].w) and puts them in the binary walk's order.
+// hi[j].xyz, ref lo[j].w) and puts them in the binary walk's order (1/d
+// finite).
 __device__ __forceinline__ void order_children(const float4 lo[4], const float4 hi[4], f3 o, f3 inv,
-                                               bool infax, float tmax, Cand4 p[4], uint32_t& tested)
+                                               float tmax, Cand4 p[4], uint32_t& tested)
 {
     Cand4 c[4];
-    float e[4][3];
+    uint32_t k[4][3];
     tested = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const uint32_t ref = __float_as_uint(lo[j].w);
         // an unused slot holds a zero box: its test runs but never counts
         const bool valid = ref != kNoChild;
-        float te = 0.0f;
-        const bool h = slab_entries(lo[j], hi[j], o, inv, tmax, e[j], te) && valid;
+        float e[3], te = 0.0f;
+        const bool h = slab_entries(lo[j], hi[j], o, inv, tmax, e, te) && valid;
         tested += valid ? 1u : 0u;
+        k[j][0] = entry_key(e[0]);
+        k[j][1] = entry_key(e[1]);
+        k[j][2] = entry_key(e[2]);
         c[j].t = te;
         c[j].ref = ref;
         c[j].hit = h ? 1u : 0u;
     }
-    const float ul = half_entry(e[0], e[1], infax), ur = half_entry(e[2], e[3], infax);
-    const float tl = c[1].ref == kNoChild ? c[0].t : ul;
-    const float tr = c[3].ref == kNoChild ? c[2].t : ur;
+    // a half whose second slot is unused is its first slot (a leaf)
+    const float tl = c[1].ref == kNoChild ? c[0].t : __uint_as_float(half_key(k[0], k[1]));
+    const float tr = c[3].ref == kNoChild ? c[2].t : __uint_as_float(half_key(k[2], k[3]));
     binary_order(c, tl, tr, p);
 }
 
@@ -612,7 +611,6 @@ struct Walk4 {
     float cur_t;
     int sp;
     uint32_t pk, pend;
-    uint32_t infax;
     BestHit H;
 };
 
@@ -620,7 +618,6 @@ struct Walk4 {
 __device__ __forceinline__ bool walk4_begin(const GeomArgs& G, Walk4& W, f3 d)
 {
     W.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    W.infax = inf_axis(W.inv) ? 1u : 0u;
     W.cur = G.root4;
     W.cur_t = 0.0f;
     W.sp = 0;
@@ -666,8 +663,7 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
             }
             Cand4 p[4];
             uint32_t tested;
-            order_children(lo, hi, o, W.inv, W.infax != 0u, W.H.found() ? W.H.best : FLT_MAX, p,
-                           tested);
+            order_children(lo, hi, o, W.inv, W.H.found() ? W.H.best : FLT_MAX, p, tested);
             if (COUNT) cnt.aabb += tested;
             const bool h0 = p[0].hit != 0u, h1 = p[1].hit != 0u, h2 = p[2].hit != 0u,
                        h3 = p[3].hit != 0u;
@@ -723,31 +719,33 @@ __device__ __forceinline__ bool mesh_raycast4_lds(const GeomArgs& G, const Lds4T
     float ovf_t[kStackCap - SK];
     Walk4 W;
     if (!walk4_begin(G, W, d)) return false;
+    if (inf_axis(W.inv)) return mesh_raycast<COUNT>(G, o, d, t_out, tri_out, flags, cnt);
     while (walk4_step<COUNT, SK>(G, L, W, o, d, ovf_ref, ovf_t, cnt)) {
     }
     return W.H.result(t_out, tri_out, flags);
 }
 
-// quad_perm moves inside a DPP quad: lane q reads lane q ^ 1 (its partner
-// in the same half of a 4-wide node).
+// quad_perm move inside a DPP quad: lane q reads lane q ^ 1 (its partner
+// in the same half of a 4-wide node).  Every lane of the quad is active
+// where it is used.
 __device__ __forceinline__ uint32_t qswap(uint32_t x)
 {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 1 | (0 << 2) | (3 << 4) | (2 << 6), 0xF, 0xF,
-                                              false);
+                                              true);
 }
 __device__ __forceinline__ float qswapf(float x) { return __uint_as_float(qswap(__float_as_uint(x))); }
 
 // Quad form of order_children's half entries: lane q holds slot q's
 // per-axis entries e, its entry te and reference ref; every lane of the
 // quad gets both halves' entry distances.
-__device__ __forceinline__ void quad_half_entries(const float e[3], float te, uint32_t ref, bool infax,
-                                                  float& tl, float& tr)
+__device__ __forceinline__ void quad_half_entries(const float e[3], float te, uint32_t ref, float& tl,
+                                                  float& tr)
 {
-    float pe[3] = {qswapf(e[0]), qswapf(e[1]), qswapf(e[2])};
+    const uint32_t k[3] = {entry_key(e[0]), entry_key(e[1]), entry_key(e[2])};
+    const uint32_t pk[3] = {qswap(k[0]), qswap(k[1]), qswap(k[2])};
     const uint32_t pref = qswap(ref);
-    const float u = half_entry(e, pe, infax);
     // on lanes 0 and 2: the half's entry (its second slot unused: the first slot's)
-    const float th = pref == kNoChild ? te : u;
+    const float th = pref == kNoChild ? te : __uint_as_float(half_key(k, pk));
     tl = qbf<0>(th);
     tr = qbf<2>(th);
 }
@@ -763,7 +761,7 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
 {
     if (G.empty) return false;
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    const bool infax = inf_axis(inv);
+    if (inf_axis(inv)) return mesh_raycast<COUNT>(G, o, d, t_out, tri_out, flags, cnt);
     uint32_t ovf_ref[kStackCap - SK];
     float ovf_t[kStackCap - SK];
     int sp = 0;
@@ -816,7 +814,7 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
                 const bool h = slab_entries(lo, hi, o, inv, H.found() ? H.best : FLT_MAX, e, te) &&
                                ref != kNoChild;
                 float tl, tr;
-                quad_half_entries(e, te, ref, infax, tl, tr);
+                quad_half_entries(e, te, ref, tl, tr);
                 const uint32_t hq = h ? 1u : 0u;
                 Cand4 c[4];
                 c[0] = Cand4{qbf<0>(te), qb<0>(ref), qb<0>(hq)};
@@ -869,7 +867,7 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
 {
     if (G.empty) return false;
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    const bool infax = inf_axis(inv);
+    if (inf_axis(inv)) return mesh_raycast<COUNT>(G, o, d, t_out, tri_out, flags, cnt);
     int sp = 0;
     uint32_t cur = G.root4;
     float cur_t = 0.0f;
@@ -913,18 +911,20 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                     cnt.aabb += __popc(valid);
                 }
                 float tl, tr;
-                quad_half_entries(e, te, ref, infax, tl, tr);
+                quad_half_entries(e, te, ref, tl, tr);
                 const uint32_t hm = (uint32_t)(__ballot(h) >> qshift) & 0xFu;
                 const int nhit = __popc(hm);
                 if (nhit > 0) {
-                    const float t0 = qbf<0>(te), t1 = qbf<1>(te), t2 = qbf<2>(te), t3 = qbf<3>(te);
-                    const bool rx = !(tl <= tr), r0 = !(t0 <= t1), r1 = !(t2 <= t3);
-                    // hit slots as bits in visiting order, then this slot's place among them
-                    uint32_t hk = 0;
-#pragma unroll
-                    for (uint32_t j = 0; j < 4; j++) hk |= ((hm >> j) & 1u) << binary_key(j, rx, r0, r1);
-                    const uint32_t key = binary_key(q, rx, r0, r1);
-                    const int rank = __popc(hk & ((1u << key) - 1u));
+                    // this slot's place among the quad's hits in visiting order:
+                    // the other half's hits come first when this half is the
+                    // farther one, the partner's when it is the nearer slot
+                    const float pte = qswapf(te);
+                    const bool odd = (q & 1u) != 0u;
+                    const bool half_far = ((q >> 1) != 0u) != !(tl <= tr);
+                    const bool slot_far = odd != !((odd ? pte : te) <= (odd ? te : pte));
+                    const uint32_t other = q < 2 ? 0xCu : 0x3u;
+                    const int rank = (half_far ? __popc(hm & other) : 0) +
+                                     ((slot_far && ((hm >> (q ^ 1u)) & 1u)) ? 1 : 0);
                     if (h && rank > 0) {
                         const int pos = sp + nhit - 1 - rank;
                         L.st_ref[pos * L.nq + L.quad] = ref;
@@ -943,6 +943,173 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
         sp--;
         cur = L.st_ref[sp * L.nq + L.quad];
         cur_t = L.st_t[sp * L.nq + L.quad];
+    }
+    return H.result(t_out, tri_out, flags);
+}
+
+// ------------------------------------------------------------------ 16-wide traversal
+//
+// A 16-lane DPP row traces one ray over the 16-wide nodes (Node16): lane r
+// loads and tests slot r (32 B; the row reads the node's four 128 B lines
+// together).  The binary walk's order over the node's four binary levels
+// comes from group entry distances: the union box of an aligned group of
+// 2, 4 or 8 slots is entered where its slots' entry keys say (entry_key),
+// reduced across the row with DPP (xor 1 and 2 by quad_perm, the other
+// quad by row_half_mirror, the other 8-group by row_mirror).  At each of
+// the four levels a slot lies in the farther of two sibling groups or in
+// the nearer one (the left one on equal entries); the hit slots of every
+// sibling group it lies behind come before it, which gives its place among
+// the node's hits, and each hit slot pushes itself at that depth of the
+// row's LDS stack (the first one becomes the current node, broadcast by an
+// OR over the row).  Culling is the binary walk's, so the leaves, their
+// order and the nearest hit are the binary walk's (see the 4-wide
+// section).  Unused slots key +inf (they have no box); a leaf's primitives
+// are tested by every quad of the row (lane r takes primitive r & 3), so
+// the quads keep identical nearest hits without crossing quads.
+
+// DPP moves inside a 16-lane row.  Every lane of the row is active where
+// they are used (the row shares its control flow), so bound_ctrl may be
+// set, which lets the compiler fold a move into the instruction using it.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+}
+
+constexpr int kDppXor1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);   // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);   // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;                           // lane i <-> 7 - i in 8
+constexpr int kDppMirror = 0x140;                               // lane i <-> 15 - i in 16
+
+// LDS working set of a 16-lane row walk: the node prefix (top 16-wide
+// nodes, breadth-first) and the stack [SK][rows per block].
+struct Row16Trav {
+    const lds_float4* nodes;   // [nc][32]
+    uint32_t nc;
+    lds_u32* st_ref;
+    lds_f32* st_t;
+    uint32_t row, nrow, r;     // row in the block, rows per block, lane in the row
+};
+
+__device__ __forceinline__ uint32_t umin3(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t umax3(uint32_t x, uint32_t y, uint32_t z)
+{
+    const uint32_t m = x > y ? x : y;
+    return m > z ? m : z;
+}
+
+// This slot's group is visited after its sibling group: right = it is the
+// right one; keys of the two groups' entries.
+__device__ __forceinline__ bool group_far(bool right, uint32_t mine, uint32_t sib)
+{
+    const uint32_t kl = right ? sib : mine, kr = right ? mine : sib;
+    return right == (kl <= kr);
+}
+
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Trav& L, f3 o, f3 d,
+                                               float& t_out, uint32_t& tri_out, uint32_t& flags,
+                                               TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    if (inf_axis(inv)) return mesh_raycast<COUNT>(G, o, d, t_out, tri_out, flags, cnt);
+    int sp = 0;
+    uint32_t cur = G.root16;
+    float cur_t = 0.0f;
+    BestHit H;
+    const uint32_t r = L.r, q = r & 3u;
+    const uint32_t rshift = (__lane_id() & 63u) & ~15u;
+    while (true) {
+        if (!(H.found() && cur_t >= H.best)) {
+            if (cur & kLeafBit) {
+                uint32_t start = cur & kLeafStartMask;
+                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+                uint32_t hit_q = 0, tri_q = 0;
+                float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+                if (q < cnt_p) {
+                    Prim p = load_prim(G.prims, start + q);
+                    tri_q = p.tri;
+                    hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+                }
+                if (COUNT && r == 0) cnt.tri += cnt_p;
+                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+            } else {
+                float4 lo, hi;
+                if (cur < L.nc) {
+                    const lds_float4* src = L.nodes + 32 * cur + 2 * r;
+                    lo = lds_ld4(src);
+                    hi = lds_ld4(src + 1);
+                } else {
+                    const OMG_GLOBAL float4* src = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes16) + cur) + 2 * r;
+                    lo = src[0];
+                    hi = src[1];
+                }
+                const uint32_t ref = __float_as_uint(lo.w);
+                const bool valid = ref != kNoChild;
+                float e[3], te = 0.0f;
+                const bool h = slab_entries(lo, hi, o, inv, H.found() ? H.best : FLT_MAX, e, te) && valid;
+                if (COUNT && r == 0) {
+                    uint32_t v = (uint32_t)(__ballot(valid) >> rshift) & 0xFFFFu;
+                    cnt.aabb += __popc(v);
+                }
+                const uint32_t hm = (uint32_t)(__ballot(h) >> rshift) & 0xFFFFu;
+                const int nhit = __popc(hm);
+                if (nhit > 0) {
+                    // per-axis keys of this slot's groups of 2, 4 and 8 (an
+                    // unused slot has no box: +inf)
+                    const uint32_t inf = 0x7F800000u;
+                    const uint32_t k0 = valid ? entry_key(e[0]) : inf;
+                    const uint32_t k1 = valid ? entry_key(e[1]) : inf;
+                    const uint32_t k2 = valid ? entry_key(e[2]) : inf;
+                    const uint32_t a0 = umin3(k0, dpp_u<kDppXor1>(k0));
+                    const uint32_t a1 = umin3(k1, dpp_u<kDppXor1>(k1));
+                    const uint32_t a2 = umin3(k2, dpp_u<kDppXor1>(k2));
+                    const uint32_t b0 = umin3(a0, dpp_u<kDppXor2>(a0));
+                    const uint32_t b1 = umin3(a1, dpp_u<kDppXor2>(a1));
+                    const uint32_t b2 = umin3(a2, dpp_u<kDppXor2>(a2));
+                    const uint32_t c0 = umin3(b0, dpp_u<kDppHalfMirror>(b0));
+                    const uint32_t c1 = umin3(b1, dpp_u<kDppHalfMirror>(b1));
+                    const uint32_t c2 = umin3(b2, dpp_u<kDppHalfMirror>(b2));
+                    const uint32_t g2 = umax3(a0, a1, a2), g4 = umax3(b0, b1, b2), g8 = umax3(c0, c1, c2);
+                    const uint32_t g1 = entry_key(te);
+                    // each level's sibling group
+                    const uint32_t s1 = dpp_u<kDppXor1>(g1), s2 = dpp_u<kDppXor2>(g2);
+                    const uint32_t s4 = dpp_u<kDppHalfMirror>(g4), s8 = dpp_u<kDppMirror>(g8);
+                    int rank = 0;
+                    rank += group_far(r & 1u, g1, s1) ? (int)((hm >> (r ^ 1u)) & 1u) : 0;
+                    rank += group_far(r & 2u, g2, s2) ? __popc(hm & (0x3u << ((r & ~1u) ^ 2u))) : 0;
+                    rank += group_far(r & 4u, g4, s4) ? __popc(hm & (0xFu << ((r & ~3u) ^ 4u))) : 0;
+                    rank += group_far(r & 8u, g8, s8) ? __popc(hm & (0xFFu << ((r & ~7u) ^ 8u))) : 0;
+                    if (h && rank > 0) {
+                        const int pos = sp + nhit - 1 - rank;
+                        L.st_ref[pos * L.nrow + L.row] = ref;
+                        L.st_t[pos * L.nrow + L.row] = te;
+                    }
+                    const bool first = h && rank == 0;
+                    uint32_t nr = first ? ref : 0u, nt = first ? __float_as_uint(te) : 0u;
+                    nr |= dpp_u<kDppXor1>(nr);
+                    nt |= dpp_u<kDppXor1>(nt);
+                    nr |= dpp_u<kDppXor2>(nr);
+                    nt |= dpp_u<kDppXor2>(nt);
+                    nr |= dpp_u<kDppHalfMirror>(nr);
+                    nt |= dpp_u<kDppHalfMirror>(nt);
+                    nr |= dpp_u<kDppMirror>(nr);
+                    nt |= dpp_u<kDppMirror>(nt);
+                    cur = nr;
+                    cur_t = __uint_as_float(nt);
+                    sp += nhit - 1;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        cur = L.st_ref[sp * L.nrow + L.row];
+        cur_t = L.st_t[sp * L.nrow + L.row];
     }
     return H.result(t_out, tri_out, flags);
 }

@@ -34,13 +34,12 @@ struct alignas(64) Node2 {
 };
 static_assert(sizeof(Node2) == 64, "Node2 is one 64 B line");
 
-// 4-wide node: the same tree collapsed so that one 128 B fetch serves up
-// to four child boxes (each internal child of a binary node is replaced by
-// its two children, largest surface area first, until four).  Children
-// keep the binary tree's left-to-right order; an unused slot has
-// ref == kNoChild.  Traversing it visits children nearest first, so the
-// nearest hit is the binary walk's; only the order among exactly equal
-// hit distances can differ, and such hits are flagged.
+// 4-wide node: the same tree collapsed two binary levels at a time, so
+// that one 128 B fetch serves four child boxes.  Slots 0-1 hold the binary
+// node's left half (its left child's two children, or that child itself
+// in slot 0 when it is a leaf, slot 1 unused), slots 2-3 the right half.
+// An unused slot has ref == kNoChild.  The fixed halves let the walk visit
+// the slots in the binary walk's order (see bvh_device.h).
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 
 struct alignas(16) Child4 {
@@ -54,6 +53,17 @@ struct alignas(128) Node4 {
     Child4 c[4];
 };
 static_assert(sizeof(Node4) == 128, "Node4 is one 128 B line");
+
+// 16-wide node: four binary levels, one slot per lane of a 16-lane row.
+// Slot group 4k..4k+3 is slot k of the 4-wide node expanded once more:
+// the four slots of the 4-wide node that slot k refers to, or the leaf in
+// slot 4k (4k+1..4k+3 unused), or four unused slots.  Hence slots [0, 8)
+// are the binary node's left child's subtree, [0, 4) its left child's,
+// and so on down to single slots.  512 B = four 128 B lines.
+struct alignas(128) Node16 {
+    Child4 c[16];
+};
+static_assert(sizeof(Node16) == 512, "Node16 is four 128 B lines");
 
 // Packed primitive (mesh.rs:43-50): vertex a, edges e1 = b - a, e2 = c - a,
 // the ORIGINAL triangle id and its material.  48 B = three 16 B loads.

@@ -88,6 +88,12 @@ struct omg_ctx {
     // regenerating kernel (0: never); its job view, items and cursor
     uint64_t route_rays = 1ull << 22;
     omg::DevBuf route_job, route_items, route_cursor;
+    // quad-per-ray plain launches hand their rays still alive after
+    // handoff_bounce bounces to the 16-lane walk (0: never); the saved
+    // states and the count / cursor words, last used on handoff_stream
+    uint32_t handoff_bounce = 8;
+    omg::DevBuf handoff, handoff_ctl;
+    hipStream_t handoff_stream = nullptr;
     bool closing = false;
     // the *_async entry points leave kernels in flight that read the staged
     // job tables (job_mats, job_ovl) and dome fan / panels; staging the same
@@ -111,9 +117,13 @@ struct omg_scene {
     std::vector<uint32_t> indices;
     std::vector<uint16_t> tri_material;
     std::vector<omg::TriInfo> tris_host;
-    omg::DevBuf nodes, prims, tris, mats, mats_raw, tri_surf, nodes4;
+    omg::DevBuf nodes, prims, tris, mats, mats_raw, tri_surf, nodes4, nodes16;
     uint32_t n_nodes4 = 0, root4 = 0, stack4 = 0;
     bool has4 = false;
+    // 16-wide nodes derived from the 4-wide ones (omg::derive16); has16
+    // when the worst-case stack fits the 16-lane walk's LDS stack
+    uint32_t n_nodes16 = 0, root16 = 0, stack16 = 0;
+    bool has16 = false;
     uint64_t device_bytes = 0;
     // scenes built on the device keep what a refit needs: the inputs, the
     // per-triangle patch counts and offsets, each patch's leaf position,
@@ -130,6 +140,12 @@ hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s);
 hipError_t launch_fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s);
 uint32_t pick_lanes(const omg_ctx* c, uint64_t n_rays);
+// lanes per ray, BVH width, LDS node prefix and stack of a plain launch
+void plain_shape(const omg_ctx* c, const omg_scene* s, const omg_trace_params* p, uint32_t n_src,
+                 bool counting, TraceArgs& A);
+// the hand-off buffers of a plain launch that plain_shape gave a hand-off
+// bounce: sized for its rays, counters cleared on `st`
+int setup_handoff(omg_ctx* c, TraceArgs& A, hipStream_t st);
 void fill_geom(const omg_scene* s, GeomArgs& g);
 int validate_params(const omg_trace_params* p, uint32_t nbands, uint32_t n_src, uint32_t n_lis,
                     uint32_t n_ovl, uint32_t* nbins);
@@ -141,6 +157,9 @@ void ctx_release(omg_ctx* c);
 void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins);
 // waits for in-flight *_async work before shared staging is overwritten
 int quiesce(omg_ctx* c);
+// derives the scene's 16-wide nodes from its resident 4-wide nodes on
+// the device (stream-ordered; waits for the stream at the end)
+int derive16(omg_scene* s, hipStream_t st);
 // host copies of the mesh and its triangle normals (scenes built on the
 // device download and compute them on first use)
 int ensure_host_mesh(omg_scene* s);

@@ -11,8 +11,10 @@ struct GeomArgs {
     const Prim* prims;
     const TriInfo* tris;
     const Node4* nodes4;     // 4-wide collapse of the same tree, or null
+    const Node16* nodes16;   // 16-wide nodes derived from nodes4, or null
     uint32_t root_ref;
     uint32_t root4;          // root of nodes4 (a leaf ref for one-leaf trees)
+    uint32_t root16;         // root of nodes16 (likewise)
     uint32_t empty;          // mesh without primitives: every ray escapes
     float box[3];            // Shoebox size (box launches)
     float root_min[3];       // bounds of the whole tree (the root's own box)
@@ -45,16 +47,35 @@ struct TraceArgs : JobView {
     uint32_t ray_count;      // rays in this launch
     uint32_t node_cache;     // BVH nodes (top levels) staged in LDS per block
     uint32_t stack_lds;      // traversal stack entries per ray kept in LDS (16 or 32)
-    uint32_t lanes;          // lanes per ray: 1, or 4 (a DPP quad)
-    uint32_t width;          // BVH width walked: 2 (paired nodes) or 4 (nodes4)
+    uint32_t lanes;          // lanes per ray: 1, 4 (a DPP quad) or 16 (a DPP row, width 16)
+    uint32_t width;          // BVH width walked: 2 (paired nodes), 4 (nodes4) or 16 (nodes16)
     uint32_t* hit_ids;       // [rays][max_bounces] hit id per bounce (0xFFFFFFFF none), or null
     float* hit_t;            // [rays][max_bounces] hit distance per bounce, or null
+    // Hand-off to the 16-lane walk: with handoff_bounce > 0 a ray still
+    // alive after that many bounces saves its state (kHandoffF4 float4s at
+    // handoff + kHandoffF4 * k, k from handoff_ctl[0]) instead of going on,
+    // and k_trace_resume16 (resume_blocks blocks, node_cache16 16-wide nodes
+    // in LDS) continues the saved rays, one 16-lane row each, taking them
+    // from the cursor handoff_ctl[1].  Both words start at zero.
+    uint32_t handoff_bounce;
+    float4* handoff;
+    uint32_t* handoff_ctl;
+    uint32_t resume_blocks;
+    uint32_t node_cache16;
 };
+
+// float4s of one saved ray state (position, distance, direction, bounce,
+// termination, hash, flags, source, ray, up to kMaxBands energies)
+constexpr uint32_t kHandoffF4 = 6;
 
 // LDS stack entries per ray of the quad 4-wide walk that never spills: used
 // when the tree's worst-case stack fits (every scene measured so far needs
 // at most 39)
 constexpr uint32_t kDeepStack = 40;
+
+// LDS stack entries per ray of the 16-lane walk over the 16-wide nodes: a
+// scene walks it when its worst-case stack fits
+constexpr uint32_t kStack16 = 96;
 
 // One unit of batched work: up to kBatchItemRays consecutive rays of one
 // source of one query.
@@ -77,22 +98,25 @@ struct BatchArgs {
 
 // Launch shape of the batched kernel (chosen by the autotuner).
 struct BatchShape {
-    uint32_t lanes;           // lanes per ray: 1 or 4
+    uint32_t lanes;           // lanes per ray: 1, 4 or 16
     uint32_t stack_lds;       // stack entries per ray in LDS (see valid_batch_shape)
     uint32_t waves_per_simd;  // resident waves per SIMD the grid is sized for
     uint32_t regen;           // 1: finished rays are replaced between bounces
-    uint32_t width;           // BVH width walked: 2 or 4 (every scene of the batch has nodes4)
+    uint32_t width;           // BVH width walked: 2, 4 or 16 (every scene of the batch has it)
 };
 
 // The compiled shapes.  Without regen: lanes 1|4 with 16|32 stack entries
 // over the binary tree, 16 over the 4-wide tree, or a quad with kDeepStack
 // entries over the 4-wide tree.  With regen: one lane per ray with 8|12|16
 // entries over either tree, or a quad per ray with 16 (binary tree) or
-// kDeepStack (4-wide tree).  A kDeepStack shape is valid only for scenes
+// kDeepStack (4-wide tree), or a 16-lane row per ray over the 16-wide
+// tree with kStack16 entries.  A kDeepStack shape is valid only for scenes
 // whose 4-wide tree needs at most kDeepStack stack entries.
 inline bool valid_batch_shape(const BatchShape& s)
 {
     if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 1) return false;
+    if (s.lanes == 16 || s.width == 16)
+        return s.lanes == 16 && s.width == 16 && s.regen == 1 && s.stack_lds == kStack16;
     if (s.width != 2 && s.width != 4) return false;
     if (s.lanes == 4 && s.width == 4 && s.stack_lds == kDeepStack) return true;
     if (s.regen && s.lanes == 4) return s.width == 2 && s.stack_lds == 16;

@@ -106,6 +106,13 @@ __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Lds4Tr
     return mesh_raycast4_lds<COUNT, SK>(G, L, o, d, t, tri, fl, c);
 }
 template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Row16Trav& L, f3 o, f3 d,
+                                                 float& t, uint32_t& tri, uint32_t& fl,
+                                                 TravCount& c)
+{
+    return mesh_raycast16<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+}
+template <bool COUNT, int SK>
 __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Quad4Trav& L, f3 o, f3 d,
                                                  float& t, uint32_t& tri, uint32_t& fl,
                                                  TravCount& c)
@@ -167,10 +174,14 @@ template <>
 struct TravOf<4, 4> {
     using type = Quad4Trav;
 };
+template <>
+struct TravOf<16, 16> {
+    using type = Row16Trav;
+};
 
-// float4s per node of width W (64 B paired node, 128 B 4-wide node)
+// float4s per node of width W (64 B paired node, 128 B 4-wide node, 512 B 16-wide node)
 template <int W>
-constexpr uint32_t node_f4() { return W == 4 ? 8u : 4u; }
+constexpr uint32_t node_f4() { return W == 16 ? 32u : W == 4 ? 8u : 4u; }
 
 template <class T>
 __device__ __forceinline__ void init_lane_trav(T& L, lds_float4* lds, uint32_t nc, uint32_t nf4, int sk)
@@ -208,6 +219,16 @@ __device__ __forceinline__ void init_trav(QuadTrav& L, lds_float4* lds, uint32_t
 __device__ __forceinline__ void init_trav(Quad4Trav& L, lds_float4* lds, uint32_t nc, int sk)
 {
     init_quad_trav(L, lds, nc, 8u, sk);
+}
+__device__ __forceinline__ void init_trav(Row16Trav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    L.nc = nc;
+    L.nodes = lds;
+    L.nrow = blockDim.x >> 4;
+    L.st_ref = reinterpret_cast<lds_u32*>(lds + 32u * nc);
+    L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nrow);
+    L.row = threadIdx.x >> 4;
+    L.r = threadIdx.x & 15u;
 }
 
 // State of one ray between bounces (tracer.rs:92-102).
@@ -372,18 +393,67 @@ __device__ __forceinline__ void ray_end(const JobView& A, const RayState<NBM>& S
     }
 }
 
+// Saves the state of a ray that goes on after the hand-off bounce (the
+// group's first lane writes it) for k_trace_resume16.
+template <int NBM>
+__device__ __forceinline__ void handoff_save(float4* out, uint32_t* ctl, const RayState<NBM>& S,
+                                             uint32_t q)
+{
+    if (q != 0) return;
+    const uint32_t k = atomicAdd(ctl, 1u);
+    float e[kMaxBands];
+#pragma unroll
+    for (int b = 0; b < kMaxBands; b++) e[b] = b < NBM ? S.energy[b] : 0.0f;
+    OMG_GLOBAL float4* o = glb(out) + (size_t)kHandoffF4 * k;
+    o[0] = make_float4(S.pos.x, S.pos.y, S.pos.z, S.dist_total);
+    o[1] = make_float4(S.dir.x, S.dir.y, S.dir.z, __uint_as_float(S.bounce));
+    o[2] = make_float4(__uint_as_float(S.term), __uint_as_float(S.hash), __uint_as_float(S.rflags),
+                       __uint_as_float(S.src));
+    o[3] = make_float4(__uint_as_float(S.ray), e[0], e[1], e[2]);
+    o[4] = make_float4(e[3], e[4], e[5], e[6]);
+    o[5] = make_float4(e[7], 0.0f, 0.0f, 0.0f);
+}
+
+template <int NBM>
+__device__ __forceinline__ void handoff_load(const float4* in, uint32_t k, RayState<NBM>& S)
+{
+    const OMG_GLOBAL float4* o = glb(in) + (size_t)kHandoffF4 * k;
+    const float4 a = o[0], b = o[1], c = o[2], d = o[3], e = o[4], f = o[5];
+    S.pos = mk(a.x, a.y, a.z);
+    S.dist_total = a.w;
+    S.dir = mk(b.x, b.y, b.z);
+    S.bounce = __float_as_uint(b.w);
+    S.term = __float_as_uint(c.x);
+    S.hash = __float_as_uint(c.y);
+    S.rflags = __float_as_uint(c.z);
+    S.src = __float_as_uint(c.w);
+    S.ray = __float_as_uint(d.x);
+    const float en[kMaxBands] = {d.y, d.z, d.w, e.x, e.y, e.z, e.w, f.x};
+#pragma unroll
+    for (int k2 = 0; k2 < NBM; k2++) S.energy[k2] = en[k2];
+}
+
 // tracer::trace (tracer.rs:80-202) for one ray of one source, start to end
-// (max_bounces >= 1).  Returns the bounce-loop count.
+// (max_bounces >= 1), or, with a hand-off bounce, until that many bounces
+// (the state is then saved for the 16-lane walk).  Returns the bounce-loop
+// count so far.
 template <int NBM, bool MESH, bool COUNT, int SK, int LANES, class Trav>
 __device__ __forceinline__ uint32_t trace_ray(const JobView& A, const Trav& L, uint32_t q,
                                               uint32_t src, uint32_t ray, uint32_t nb,
-                                              uint32_t* hit_ids, float* hit_t, TravCount& tc)
+                                              uint32_t* hit_ids, float* hit_t, TravCount& tc,
+                                              uint32_t handoff_bounce = 0u,
+                                              float4* handoff = nullptr, uint32_t* ctl = nullptr)
 {
     RayState<NBM> S;
     ray_begin(A, src, ray, S);
-    while (ray_step<NBM, MESH, COUNT, SK, LANES>(A, L, q, nb, S, hit_ids, hit_t, tc)) {
+    bool cont;
+    while ((cont = ray_step<NBM, MESH, COUNT, SK, LANES>(A, L, q, nb, S, hit_ids, hit_t, tc))) {
+        if (S.bounce == handoff_bounce) break;
     }
-    ray_end(A, S, q);
+    if (cont)
+        handoff_save(handoff, ctl, S, q);
+    else
+        ray_end(A, S, q);
     return S.bounce;
 }
 
@@ -418,8 +488,9 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
     typename TravOf<LANES, W>::type L;
     init_trav(L, (lds_float4*)lds, MESH ? A.node_cache : 0u, SK);
     if (MESH) {
-        const float4* src4 = W == 4 ? reinterpret_cast<const float4*>(A.g.nodes4)
-                                    : reinterpret_cast<const float4*>(A.g.nodes);
+        const float4* src4 = W == 16 ? reinterpret_cast<const float4*>(A.g.nodes16)
+                             : W == 4 ? reinterpret_cast<const float4*>(A.g.nodes4)
+                                      : reinterpret_cast<const float4*>(A.g.nodes);
         for (uint32_t i = threadIdx.x; i < node_f4<W>() * A.node_cache; i += blockDim.x)
             lds[i] = src4[i];
         __syncthreads();
@@ -435,7 +506,9 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
         uint32_t ray = gid - src * A.n_rays;
         TravCount tc;
         uint32_t iters = trace_ray<NBM, MESH, COUNT, SK, LANES>(A, L, q, src, ray, (uint32_t)NBM,
-                                                                A.hit_ids, A.hit_t, tc);
+                                                                A.hit_ids, A.hit_t, tc,
+                                                                A.handoff_bounce, A.handoff,
+                                                                A.handoff_ctl);
         if (COUNT && q == 0) {
             aabb_acc = tc.aabb;
             tri_acc = tc.tri;
@@ -443,6 +516,59 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
         }
     }
     if (COUNT && A.counters) flush_counters(A.counters, aabb_acc, tri_acc, iters_acc);
+}
+
+// Second stage of a handed-off plain launch: the rays that k_trace saved
+// after its hand-off bounce, each continued by a 16-lane row over the
+// 16-wide nodes until it ends (a ray's later bounces are its own: the
+// counter RNG and the saved state make them the same as in one launch).
+// Persistent: the grid is sized to the resident wave slots and a row whose
+// ray has ended takes the next saved ray from the cursor; dead rows of a
+// wave are ranked by a ballot over their first lanes.  The saved rays are
+// the long-lived ones, so a row per ray cuts each bounce's dependent chain
+// of node fetches (four binary levels per fetch) where the early, crowded
+// bounces use a quad per ray.
+template <int NBM, int SK>
+__global__ __launch_bounds__(256) void k_trace_resume16(TraceArgs A)
+{
+    extern __shared__ float4 lds[];
+    Row16Trav L;
+    init_trav(L, (lds_float4*)lds, A.node_cache16, SK);
+    for (uint32_t i = threadIdx.x; i < 32u * A.node_cache16; i += blockDim.x)
+        lds[i] = reinterpret_cast<const float4*>(A.g.nodes16)[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = threadIdx.x & 15u;
+    const uint64_t before = (1ull << (lane & ~15u)) - 1ull;
+    const uint32_t n = *reinterpret_cast<const volatile uint32_t*>(A.handoff_ctl);
+    RayState<NBM> S;
+    uint32_t live = 0, more = 1;
+    for (;;) {
+        // every lane of the wave is active here
+        const uint64_t need = __ballot(live == 0u && r == 0u);
+        if (need != 0 && more) {
+            const uint32_t k = (uint32_t)__popcll(need);
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(A.handoff_ctl + 1, k);
+            base = __builtin_amdgcn_readfirstlane(base);
+            const uint32_t rank = (uint32_t)__popcll(need & before);
+            if (live == 0u && base + rank < n) {
+                handoff_load(A.handoff, base + rank, S);
+                live = 1u;
+            }
+            if (base + k >= n) more = 0;
+        }
+        if (__ballot(live != 0u) == 0) break;
+        if (live) {
+            TravCount tc;
+            bool cont = ray_step<NBM, true, false, SK, 16>(A, L, r, (uint32_t)NBM, S, A.hit_ids,
+                                                          A.hit_t, tc);
+            if (!cont) {
+                ray_end(A, S, r);
+                live = 0u;
+            }
+        }
+    }
 }
 
 // Batched persistent launch: the grid is sized to the resident wave slots and
@@ -561,22 +687,24 @@ void k_trace_batch_regen(BatchArgs B)
     if (COUNT && B.counters) flush_counters(B.counters, aabb_acc, tri_acc, iters_acc);
 }
 
-// Quad variant of the regenerating launch: a DPP quad runs each ray (the
-// 4-wide walk tests a node's four children on the quad's lanes, the binary
-// walk splits the pair tests; the lanes share the listeners), and a quad
-// whose ray has ended takes the next ray of its wave's item.  Ranks go to
-// the dead quads by the first lane of each quad: a quad's rank is the number
-// of dead quads before it in the wave.  With the 4-wide tree the whole
-// traversal stack (kDeepStack entries per quad) lives in LDS.
-template <int NBM, bool COUNT, int SK, int W>
-__global__ __launch_bounds__(256) void k_trace_batch_regen4(BatchArgs B)
+// Multi-lane variant of the regenerating launch: a group of LANES lanes
+// runs each ray (a DPP quad on the 4-wide or binary tree: the 4-wide walk
+// tests a node's four children on the quad's lanes, the binary walk splits
+// the pair tests; a 16-lane row on the 16-wide tree, one slot per lane;
+// the lanes share the listeners), and a group whose ray has ended takes
+// the next ray of its wave's item.  Ranks go to the dead groups by the
+// first lane of each group: a group's rank is the number of dead groups
+// before it in the wave.  On the 4-wide and 16-wide trees the whole
+// traversal stack (kDeepStack / kStack16 entries per group) lives in LDS.
+template <int NBM, bool COUNT, int SK, int W, int LANES>
+__global__ __launch_bounds__(256) void k_trace_batch_regenL(BatchArgs B)
 {
     extern __shared__ float4 lds[];
-    typename TravOf<4, W>::type L;
+    typename TravOf<LANES, W>::type L;
     init_trav(L, (lds_float4*)lds, 0u, SK);
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t q = threadIdx.x & 3u;
-    const uint64_t before = (1ull << (lane & ~3u)) - 1ull;  // lanes ahead of this quad
+    const uint32_t q = threadIdx.x & (LANES - 1u);
+    const uint64_t before = (1ull << (lane & ~(LANES - 1u))) - 1ull;  // lanes ahead of this group
     RayState<NBM> S;
     uint32_t live = 0, qi = 0;
     uint32_t it_q = 0, it_src = 0, it_next = 0, it_end = 0, more = 1;
@@ -614,7 +742,7 @@ __global__ __launch_bounds__(256) void k_trace_batch_regen4(BatchArgs B)
         if (live) {
             const JobView& J = B.jobs[qi];
             TravCount tc;
-            bool cont = ray_step<NBM, true, COUNT, SK, 4>(J, L, q, J.nbands, S, nullptr, nullptr, tc);
+            bool cont = ray_step<NBM, true, COUNT, SK, LANES>(J, L, q, J.nbands, S, nullptr, nullptr, tc);
             if (COUNT && q == 0) {
                 aabb_acc += tc.aabb;
                 tri_acc += tc.tri;
@@ -688,6 +816,10 @@ template <int NBM, bool MESH, bool COUNT>
 static hipError_t launch_trace_sk(const TraceArgs& A, hipStream_t s)
 {
     if (!MESH) return launch_trace_t<NBM, MESH, COUNT, 1>(A, s);
+    if (A.width == 16) {
+        if (!A.g.nodes16 || A.lanes != 16) return hipErrorInvalidValue;
+        return launch_trace_l<NBM, true, COUNT, kStack16, 16, 16>(A, s);
+    }
     if (A.width == 4) {
         if (!A.g.nodes4) return hipErrorInvalidValue;
         return launch_trace_w4<NBM, COUNT>(A, s);
@@ -712,10 +844,38 @@ static hipError_t launch_trace_nb(const TraceArgs& A, hipStream_t s)
     }
 }
 
+template <int NBM>
+static hipError_t launch_resume16_n(const TraceArgs& A, hipStream_t s)
+{
+    size_t lds = (size_t)A.node_cache16 * sizeof(Node16) + (size_t)kStack16 * (kTraceBlock / 16) * 8;
+    hipLaunchKernelGGL((k_trace_resume16<NBM, kStack16>), dim3(A.resume_blocks), dim3(kTraceBlock),
+                       lds, s, A);
+    return hipGetLastError();
+}
+
+static hipError_t launch_resume16(const TraceArgs& A, hipStream_t s)
+{
+    switch (A.nbands) {
+    case 1: return launch_resume16_n<1>(A, s);
+    case 2: return launch_resume16_n<2>(A, s);
+    case 3: return launch_resume16_n<3>(A, s);
+    case 4: return launch_resume16_n<4>(A, s);
+    case 5: return launch_resume16_n<5>(A, s);
+    case 6: return launch_resume16_n<6>(A, s);
+    case 7: return launch_resume16_n<7>(A, s);
+    case 8: return launch_resume16_n<8>(A, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s)
 {
-    if (mesh) return count ? launch_trace_nb<true, true>(A, s) : launch_trace_nb<true, false>(A, s);
-    return count ? launch_trace_nb<false, true>(A, s) : launch_trace_nb<false, false>(A, s);
+    hipError_t e;
+    if (mesh) e = count ? launch_trace_nb<true, true>(A, s) : launch_trace_nb<true, false>(A, s);
+    else e = count ? launch_trace_nb<false, true>(A, s) : launch_trace_nb<false, false>(A, s);
+    if (e != hipSuccess || !mesh || count || A.handoff_bounce == 0) return e;
+    if (!A.g.nodes16 || !A.handoff || !A.handoff_ctl || A.resume_blocks == 0) return hipErrorInvalidValue;
+    return launch_resume16(A, s);
 }
 
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s)
@@ -761,12 +921,12 @@ static hipError_t launch_batch_regen(const BatchArgs& B, uint32_t width, uint32_
     return hipGetLastError();
 }
 
-template <int NBM, bool COUNT, int SK, int W>
-static hipError_t launch_batch_regen4(const BatchArgs& B, uint32_t blocks, hipStream_t s)
+template <int NBM, bool COUNT, int SK, int W, int LANES>
+static hipError_t launch_batch_regenL(const BatchArgs& B, uint32_t blocks, hipStream_t s)
 {
-    size_t lds = (size_t)SK * (kTraceBlock / 4) * 8;
-    hipLaunchKernelGGL((k_trace_batch_regen4<NBM, COUNT, SK, W>), dim3(blocks), dim3(kTraceBlock),
-                       lds, s, B);
+    size_t lds = (size_t)SK * (kTraceBlock / LANES) * 8;
+    hipLaunchKernelGGL((k_trace_batch_regenL<NBM, COUNT, SK, W, LANES>), dim3(blocks),
+                       dim3(kTraceBlock), lds, s, B);
     return hipGetLastError();
 }
 
@@ -774,9 +934,10 @@ template <int NBM, bool COUNT>
 static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint32_t blocks,
                                   hipStream_t s)
 {
+    if (sh.regen && sh.lanes == 16) return launch_batch_regenL<NBM, COUNT, kStack16, 16, 16>(B, blocks, s);
     if (sh.regen && sh.lanes == 4)
-        return sh.width == 4 ? launch_batch_regen4<NBM, COUNT, kDeepStack, 4>(B, blocks, s)
-                             : launch_batch_regen4<NBM, COUNT, 16, 2>(B, blocks, s);
+        return sh.width == 4 ? launch_batch_regenL<NBM, COUNT, kDeepStack, 4, 4>(B, blocks, s)
+                             : launch_batch_regenL<NBM, COUNT, 16, 2, 4>(B, blocks, s);
     if (sh.regen) {
         switch (sh.stack_lds) {
         case 8: return launch_batch_regen<NBM, COUNT, 8>(B, sh.width, blocks, s);
@@ -836,8 +997,9 @@ hipError_t launch_trace_routed(const TraceArgs& A, uint32_t nbm, uint32_t width,
     B.next_item = d_cursor;
     B.counters = A.counters;
     // the batched kernel's default shape: one lane per ray, 12 LDS stack
-    // entries, 6 waves per SIMD, rays regenerated between bounces
-    BatchShape sh{1u, 12u, 6u, 1u, width};
+    // entries, 6 waves per SIMD, rays regenerated between bounces; a 16-lane
+    // row per ray over the 16-wide tree when that is the width walked
+    BatchShape sh = width == 16 ? BatchShape{16u, kStack16, 6u, 1u, 16u} : BatchShape{1u, 12u, 6u, 1u, width};
     return launch_trace_batch(B, nbm, sh, n_cu, count, s, 0);
 }
 

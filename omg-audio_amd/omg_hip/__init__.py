@@ -110,6 +110,12 @@ class Context:
         regenerating kernel (0 = never; omg_set_route_rays)."""
         check(lib().omg_set_route_rays(self.h, min_rays))
 
+    def set_handoff_bounce(self, bounce=8):
+        """Quad-per-ray plain launches continue the rays still alive after
+        `bounce` bounces on a 16-lane row each (0 = never;
+        omg_set_handoff_bounce)."""
+        check(lib().omg_set_handoff_bounce(self.h, bounce))
+
     def set_launch_config(self, node_cache=512, stack_lds=16, lanes=0, width=4):
         """Tracer launch shape: top BVH nodes staged in LDS (64 B units), LDS
         stack depth, lanes per ray (1, 4, or 0 = chosen by launch size), BVH
@@ -190,7 +196,7 @@ class Scene:
         nodes4 = np.zeros((t.n_nodes4, 32), np.float32)
         check(lib().omg_scene_export(self.h, _p(prims), t.n_prims, _p(nodes), t.n_nodes,
                                      _p(nodes4), t.n_nodes4, C.byref(t)))
-        summary = {k: getattr(t, k) for k, _ in SceneTree._fields_ if k not in ("pad",)}
+        summary = {k: getattr(t, k) for k, _ in SceneTree._fields_ if k not in ("pad", "pad16")}
         summary["root_min"] = list(t.root_min)
         summary["root_max"] = list(t.root_max)
         return prims, nodes, nodes4, summary

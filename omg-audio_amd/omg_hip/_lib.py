@@ -85,7 +85,9 @@ class SceneTree(C.Structure):
     _fields_ = [("n_prims", C.c_uint32), ("n_nodes", C.c_uint32), ("n_nodes4", C.c_uint32),
                 ("root_ref", C.c_uint32), ("root4", C.c_uint32), ("stack4", C.c_uint32),
                 ("has4", C.c_uint32), ("max_depth", C.c_uint32), ("empty", C.c_uint32),
-                ("pad", C.c_uint32), ("root_min", C.c_float * 3), ("root_max", C.c_float * 3)]
+                ("pad", C.c_uint32), ("root_min", C.c_float * 3), ("root_max", C.c_float * 3),
+                ("n_nodes16", C.c_uint32), ("stack16", C.c_uint32), ("has16", C.c_uint32),
+                ("pad16", C.c_uint32)]
 
 
 class DomePanel(C.Structure):
@@ -285,6 +287,7 @@ SIGNATURES = [
     ("omg_solve_batch", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp,
                              _vp]),
     ("omg_set_route_rays", _i, [_vp, C.c_uint64]),
+    ("omg_set_handoff_bounce", _i, [_vp, C.c_uint32]),
     ("omg_paths_extract_edges", _i, [_vp, _u32, _vp, _u32, _f32, _u32, _vp, _u32,
                                      C.POINTER(_u32)]),
     ("omg_paths_create", _i, [_vp, _u32, C.POINTER(_vp)]),

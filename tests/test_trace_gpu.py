@@ -139,15 +139,19 @@ def test_launch_shapes_give_identical_rays(ctx, oracle, shape):
 
 
 @pytest.mark.parametrize("scene_name", ["city", "c1"])
-@pytest.mark.parametrize("shape", [(0, 16, 1), (256, 16, 1), (0, 16, 4), (256, 16, 4), (256, 32, 4)])
+@pytest.mark.parametrize("shape", [(0, 16, 1, 4, 0), (256, 16, 1, 4, 0), (0, 16, 4, 4, 0),
+                                   (256, 16, 4, 4, 0), (256, 32, 4, 4, 0), (256, 16, 4, 4, 1),
+                                   (256, 16, 4, 4, 8), (0, 16, 16, 16, 0), (512, 16, 16, 16, 0)])
 def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape):
-    """The 4-wide collapse visits the binary walk's leaves in the binary
-    walk's order, so every ray - flagged near-edge or exact-tie hits
-    included - has the oracle's record, and every bounce the binary walk's
-    hit id and distance.  Shapes: one lane per ray, a quad with the whole
-    stack in LDS (16 requested), a quad with 16 LDS entries and the rest in
-    scratch (32 requested); with and without the LDS node prefix."""
-    nc, sk, lanes = shape
+    """The 4-wide and 16-wide nodes are visited in the binary walk's order,
+    so every ray - flagged near-edge or exact-tie hits included - has the
+    oracle's record, and every bounce the binary walk's hit id and
+    distance.  Shapes (LDS node prefix, stack, lanes, width, hand-off
+    bounce): one lane per ray; a quad with the whole stack in LDS (16
+    requested); a quad with 16 LDS entries and the rest in scratch (32
+    requested); a quad handing its rays to 16-lane rows after 1 or 8
+    bounces; a 16-lane row per ray from the start."""
+    nc, sk, lanes, width, handoff = shape
     if scene_name == "city":
         mb, bounds, boxes = scenes.city_block(seed=5, n_buildings=8, target_tris=6000)
         src = np.vstack([scenes.indoor_points(51, 2, boxes),
@@ -156,14 +160,19 @@ def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape)
     else:
         mb, s1, l1, _ = scenes.shoebox(seed=1)
         src, lis = np.array([s1]), np.array([l1])
-    _say(f"case width 4 {scene_name} node_cache={nc} stack_lds={sk} lanes={lanes}")
+    _say(f"case wide walk {scene_name} node_cache={nc} stack_lds={sk} lanes={lanes} width={width} "
+         f"handoff={handoff}")
     kw = dict(n_rays=1000, nbands=3, seed=41, bin_dt=0.001, max_time=2.0)
     scene = H.Scene.from_builder(ctx, mb)
+    tree = scene.export()[3]
+    assert tree["has4"] and tree["has16"]
     try:
-        ctx.set_launch_config(nc, sk, lanes, 4)
+        ctx.set_launch_config(nc, sk, lanes, width)
+        ctx.set_handoff_bounce(handoff)
         gpu, ids, ts = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)
     finally:
         ctx.set_launch_config()
+        ctx.set_handoff_bounce()
     try:
         ctx.set_launch_config(512, 16, 0, 2)
         ref, ids2, ts2 = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)
@@ -178,7 +187,7 @@ def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape)
     assert not diff.any(), f"{int(diff.sum())} rays differ ({flagged} flagged in the oracle)"
     assert np.array_equal(ids, ids2) and np.array_equal(ts, ts2)
     _say(f"  every ray identical hit by hit ({flagged} carry a near-edge or tie flag)")
-    compare(gpu, outs, 1000, label=f"width 4 {scene_name}")
+    compare(gpu, outs, 1000, label=f"width {width} {scene_name}")
 
 
 def test_overlays_six_bands(ctx, oracle):
