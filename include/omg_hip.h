@@ -134,7 +134,7 @@ int omg_set_route_rays(omg_ctx *ctx, uint64_t min_rays);
  * 16-lane row over the 16-wide nodes: the first bounces keep all rays in
  * flight four lanes each, the long-lived rays then walk four binary levels
  * per node fetch.  The same rays and per-ray records; echograms equal up
- * to the order of their f32 sums.  Default 8; 0 = off.  No reference
+ * to the order of their f32 sums.  Default 4; 0 = off.  No reference
  * counterpart. */
 int omg_set_handoff_bounce(omg_ctx *ctx, uint32_t bounce);
 

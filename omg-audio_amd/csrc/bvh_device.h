@@ -355,15 +355,59 @@ struct QuadTrav {
     uint32_t quad, nq, q;      // quad index in the block, quads per block, lane in quad
 };
 
-template <int J>
-__device__ __forceinline__ void fold_prim(uint32_t cnt_p, uint32_t hit_q, float t_q, uint32_t tri_q,
+// Quad reductions by DPP (xor 1, then xor 2 inside the quad).  Every lane
+// of the quad is active where they are used.
+constexpr int kDppXor1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);   // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);   // quad_perm [2,3,0,1]
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t quad_min(uint32_t x)
+{
+    const uint32_t y = dpp_u<kDppXor1>(x);
+    x = x < y ? x : y;
+    const uint32_t z = dpp_u<kDppXor2>(x);
+    return x < z ? x : z;
+}
+__device__ __forceinline__ uint32_t quad_or(uint32_t x)
+{
+    x |= dpp_u<kDppXor1>(x);
+    return x | dpp_u<kDppXor2>(x);
+}
+
+// A leaf tested by a quad (lane q: primitive q of the leaf when q < cnt_p,
+// hit_q / t_q / u_q / v_q its result) folded into H exactly as offering the
+// primitives one by one in leaf order does (BestHit::offer_if): with m the
+// least hit distance, the first primitive at m wins when m < best (and the
+// tie flag is then whether another triangle is also at m), otherwise H
+// stays and its tie flag also records a hit of another triangle at best.
+// Hit distances exceed 1e-5, so positive floats order as their bits.
+__device__ __forceinline__ void fold_leaf(uint32_t cnt_p, uint32_t hit_q, float t_q, uint32_t tri_q,
                                           float u_q, float v_q, BestHit& H)
 {
-    uint32_t h = qb<J>(hit_q);
-    float t = qbf<J>(t_q);
-    uint32_t tri = qb<J>(tri_q);
-    float u = qbf<J>(u_q), v = qbf<J>(v_q);
-    H.offer_if(((uint32_t)J < cnt_p) & (h != 0u), t, tri, u, v);
+    const uint32_t q = __lane_id() & 3u;
+    const bool hit = (q < cnt_p) & (hit_q != 0u);
+    const uint32_t key = hit ? __float_as_uint(t_q) : 0xFFFFFFFFu;
+    const uint32_t m = quad_min(key);
+    if (m == 0xFFFFFFFFu) return;
+    const float tm = __uint_as_float(m);
+    const bool is_m = key == m;
+    const uint32_t w = quad_min(is_m ? q : 4u);
+    const bool is_w = q == w;
+    const uint32_t wtri = quad_or(is_w ? tri_q : 0u);
+    const float wu = __uint_as_float(quad_or(is_w ? __float_as_uint(u_q) : 0u));
+    const float wv = __uint_as_float(quad_or(is_w ? __float_as_uint(v_q) : 0u));
+    const uint32_t tie_m = quad_or((is_m & !is_w & (tri_q != wtri)) ? 1u : 0u);
+    const uint32_t tie_b = quad_or((hit & (t_q == H.best) & (tri_q != H.btri)) ? 1u : 0u);
+    const bool better = !H.found() | (tm < H.best);
+    H.best = better ? tm : H.best;
+    H.btri = better ? wtri : H.btri;
+    H.bu = better ? wu : H.bu;
+    H.bv = better ? wv : H.bv;
+    H.tie = better ? tie_m : (H.tie | tie_b);
 }
 
 template <bool COUNT, int SK>
@@ -393,10 +437,7 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
                     hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
                 }
                 if (COUNT && q == 0) cnt.tri += cnt_p;
-                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
             } else {
                 // lane 0: left child (q[0], q[1]); lane 1: right child (q[2], q[3])
                 uint32_t half = q & 1u;
@@ -794,10 +835,7 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
                     hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
                 }
                 if (COUNT && q == 0) cnt.tri += cnt_p;
-                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
             } else {
                 float4 lo, hi;
                 if (cur < L.nc) {
@@ -887,10 +925,7 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                     hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
                 }
                 if (COUNT && q == 0) cnt.tri += cnt_p;
-                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
             } else {
                 float4 lo, hi;
                 if (cur < L.nc) {
@@ -967,17 +1002,9 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
 // are tested by every quad of the row (lane r takes primitive r & 3), so
 // the quads keep identical nearest hits without crossing quads.
 
-// DPP moves inside a 16-lane row.  Every lane of the row is active where
-// they are used (the row shares its control flow), so bound_ctrl may be
-// set, which lets the compiler fold a move into the instruction using it.
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_u(uint32_t x)
-{
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
-}
-
-constexpr int kDppXor1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);   // quad_perm [1,0,3,2]
-constexpr int kDppXor2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);   // quad_perm [2,3,0,1]
+// DPP moves inside a 16-lane row (dpp_u; every lane of the row is active
+// where they are used, the row sharing its control flow): besides the quad
+// moves, the other quad of an 8-group and the other 8-group.
 constexpr int kDppHalfMirror = 0x141;                           // lane i <-> 7 - i in 8
 constexpr int kDppMirror = 0x140;                               // lane i <-> 15 - i in 16
 
@@ -1033,10 +1060,7 @@ __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Tra
                     hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
                 }
                 if (COUNT && r == 0) cnt.tri += cnt_p;
-                fold_prim<0>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<1>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<2>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-                fold_prim<3>(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
             } else {
                 float4 lo, hi;
                 if (cur < L.nc) {

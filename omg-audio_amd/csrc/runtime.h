@@ -91,7 +91,7 @@ struct omg_ctx {
     // quad-per-ray plain launches hand their rays still alive after
     // handoff_bounce bounces to the 16-lane walk (0: never); the saved
     // states and the count / cursor words, last used on handoff_stream
-    uint32_t handoff_bounce = 8;
+    uint32_t handoff_bounce = 4;
     omg::DevBuf handoff, handoff_ctl;
     hipStream_t handoff_stream = nullptr;
     bool closing = false;

@@ -110,7 +110,7 @@ class Context:
         regenerating kernel (0 = never; omg_set_route_rays)."""
         check(lib().omg_set_route_rays(self.h, min_rays))
 
-    def set_handoff_bounce(self, bounce=8):
+    def set_handoff_bounce(self, bounce=4):
         """Quad-per-ray plain launches continue the rays still alive after
         `bounce` bounces on a 16-lane row each (0 = never;
         omg_set_handoff_bounce)."""
