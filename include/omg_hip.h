@@ -130,13 +130,16 @@ int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds,
 int omg_set_route_rays(omg_ctx *ctx, uint64_t min_rays);
 /* Plain launches that trace a quad per ray over the 4-wide tree (below the
  * routing threshold, with the whole stack in LDS) hand every ray still
- * alive after `bounce` bounces to a second kernel that continues it on a
- * 16-lane row over the 16-wide nodes: the first bounces keep all rays in
- * flight four lanes each, the long-lived rays then walk four binary levels
- * per node fetch.  The same rays and per-ray records; echograms equal up
- * to the order of their f32 sums.  Default 4; 0 = off.  No reference
+ * alive after `bounce` bounces to a second, persistent kernel that
+ * continues it on a 16-lane row over the 16-wide nodes: the first bounces
+ * keep all rays in flight four lanes each, the long-lived rays then walk
+ * four binary levels per node fetch.  node_cache16: 16-wide nodes of the
+ * top levels staged in LDS per block of that kernel (512 B each);
+ * waves_per_simd: resident waves per SIMD its grid is sized for (1..8).
+ * The same rays and per-ray records; echograms equal up to the order of
+ * their f32 sums.  Defaults 4, 32, 6; bounce 0 = off.  No reference
  * counterpart. */
-int omg_set_handoff_bounce(omg_ctx *ctx, uint32_t bounce);
+int omg_set_handoff(omg_ctx *ctx, uint32_t bounce, uint32_t node_cache16, uint32_t waves_per_simd);
 
 /* ------------------------------------------------------------ scenes */
 

@@ -101,10 +101,16 @@ extern "C" int omg_set_route_rays(omg_ctx* c, uint64_t min_rays)
     return OMG_OK;
 }
 
-extern "C" int omg_set_handoff_bounce(omg_ctx* c, uint32_t bounce)
+extern "C" int omg_set_handoff(omg_ctx* c, uint32_t bounce, uint32_t node_cache16,
+                               uint32_t waves_per_simd)
 {
     if (!c) return fail(OMG_ERR_INVALID, "bad argument");
+    if (waves_per_simd < 1 || waves_per_simd > 8) return fail(OMG_ERR_INVALID, "waves_per_simd must be 1..8");
+    if ((size_t)node_cache16 * sizeof(Node16) + (size_t)kStack16 * 16 * 8 > 64 * 1024)
+        return fail(OMG_ERR_CAP, "LDS working set above 64 KiB");
     c->handoff_bounce = bounce;
+    c->handoff_nc = node_cache16;
+    c->handoff_wps = waves_per_simd;
     return OMG_OK;
 }
 
@@ -153,8 +159,8 @@ void omg::plain_shape(const omg_ctx* c, const omg_scene* s, const omg_trace_para
                 // the rays still alive after the first bounces continue on a
                 // 16-lane row each (see k_trace_resume16)
                 A.handoff_bounce = c->handoff_bounce;
-                A.resume_blocks = c->n_cu * 6u;
-                A.node_cache16 = std::min(s->n_nodes16, 16u);
+                A.resume_blocks = c->n_cu * c->handoff_wps;
+                A.node_cache16 = std::min(s->n_nodes16, c->handoff_nc);
             }
         }
     }

@@ -535,21 +535,23 @@ __device__ __forceinline__ Cand4 csel(bool c, const Cand4& a, const Cand4& b)
 }
 
 // Slab test of one slot box that also returns the per-axis entries (the
-// lesser of the two plane distances, as the slab test takes them).
+// lesser of the two plane distances, as the slab test takes them).  For a
+// finite 1/d (the wide walks hand other rays to the binary walk) the plane
+// distances are numbers or infinities, never NaN, so the swap of the
+// reference's test is a min / max and the result is ray_aabb's.
 __device__ __forceinline__ bool slab_entries(const float4& lo, const float4& hi, f3 o, f3 inv,
                                              float tmax, float e[3], float& t_enter)
 {
     const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
-    float t0 = 0.0f, t1 = tmax;
+    float ex[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        float ta = (bmin[a] - get(o, a)) * get(inv, a), tb = (bmax[a] - get(o, a)) * get(inv, a);
-        const bool sw = ta > tb;
-        const float en = sw ? tb : ta, ex = sw ? ta : tb;
-        e[a] = en;
-        t0 = fmaxf(t0, en);
-        t1 = fminf(t1, ex);
+        const float ta = (bmin[a] - get(o, a)) * get(inv, a), tb = (bmax[a] - get(o, a)) * get(inv, a);
+        e[a] = fminf(ta, tb);
+        ex[a] = fmaxf(ta, tb);
     }
+    const float t0 = fmaxf(fmaxf(fmaxf(0.0f, e[0]), e[1]), e[2]);
+    const float t1 = fminf(fminf(fminf(tmax, ex[0]), ex[1]), ex[2]);
     t_enter = t0;
     return !(t0 > t1);
 }

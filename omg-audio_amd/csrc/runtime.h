@@ -92,6 +92,8 @@ struct omg_ctx {
     // handoff_bounce bounces to the 16-lane walk (0: never); the saved
     // states and the count / cursor words, last used on handoff_stream
     uint32_t handoff_bounce = 4;
+    uint32_t handoff_nc = 32;           // 16-wide nodes in LDS per block of the resume kernel
+    uint32_t handoff_wps = 6;           // its resident waves per SIMD
     omg::DevBuf handoff, handoff_ctl;
     hipStream_t handoff_stream = nullptr;
     bool closing = false;

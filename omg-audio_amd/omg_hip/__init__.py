@@ -110,11 +110,12 @@ class Context:
         regenerating kernel (0 = never; omg_set_route_rays)."""
         check(lib().omg_set_route_rays(self.h, min_rays))
 
-    def set_handoff_bounce(self, bounce=4):
+    def set_handoff(self, bounce=4, node_cache16=32, waves_per_simd=6):
         """Quad-per-ray plain launches continue the rays still alive after
-        `bounce` bounces on a 16-lane row each (0 = never;
-        omg_set_handoff_bounce)."""
-        check(lib().omg_set_handoff_bounce(self.h, bounce))
+        `bounce` bounces on a 16-lane row each (0 = never), in a persistent
+        kernel with node_cache16 top 16-wide nodes in LDS per block, sized
+        for waves_per_simd resident waves per SIMD (omg_set_handoff)."""
+        check(lib().omg_set_handoff(self.h, bounce, node_cache16, waves_per_simd))
 
     def set_launch_config(self, node_cache=512, stack_lds=16, lanes=0, width=4):
         """Tracer launch shape: top BVH nodes staged in LDS (64 B units), LDS

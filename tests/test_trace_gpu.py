@@ -168,11 +168,11 @@ def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape)
     assert tree["has4"] and tree["has16"]
     try:
         ctx.set_launch_config(nc, sk, lanes, width)
-        ctx.set_handoff_bounce(handoff)
+        ctx.set_handoff(handoff)
         gpu, ids, ts = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)
     finally:
         ctx.set_launch_config()
-        ctx.set_handoff_bounce()
+        ctx.set_handoff()
     try:
         ctx.set_launch_config(512, 16, 0, 2)
         ref, ids2, ts2 = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)
