@@ -443,12 +443,24 @@ typedef struct {
 /* SurfaceTable::build (pt_mesh.rs:51-80): surfaces of the scene's mesh;
  * *n_out receives the count (the first overlay-face id). */
 int omg_surface_table(omg_scene *scene, omg_surface *out, uint32_t cap, uint32_t *n_out);
+/* SurfaceTable::append_box (pt_mesh.rs:87-106), host only: the six faces of
+ * an overlay box (furniture) as rect-bounded reflectors with its material,
+ * out6[0..5] = x min, x max, y min, y max, z min, z max -- appended after
+ * the table, the first one's index is the `base` of omg_discover. */
+int omg_surface_append_box(const float mn[3], const float mx[3], const omg_material *mat,
+                           uint32_t nbands, omg_surface *out6);
 
 /* WorldDiscovery::discover (crates/omg-scene/src/early_world.rs:24-26;
- * mesh_chains, pt_mesh.rs:375-457) without overlay boxes: chains as
- * [3] u16 + order, deduplicated in first-seen order. */
-int omg_discover(omg_scene *scene, const float listener[3], uint32_t n_rays, uint32_t rot,
-                 uint16_t *chains, uint8_t *orders, uint32_t cap, uint32_t *n_out);
+ * mesh_chains, pt_mesh.rs:375-457; GpuWorldDiscovery passes the furniture
+ * boxes when furniture is on, crates/omg-gpu/src/lib.rs:832-843): the
+ * rotating fan over the mesh and the overlay boxes [n_boxes][6] (min[3],
+ * max[3]; at most 64, refused above), whose faces reflect too with surface
+ * ids base + box * 6 + face (face = axis * 2 + 0 for the min side, 1 for
+ * the max side, as SurfaceTable::append_box numbers them).  Chains as [3]
+ * u16 + order, deduplicated in first-seen order. */
+int omg_discover(omg_scene *scene, const float *boxes, uint32_t n_boxes, uint16_t base,
+                 const float listener[3], uint32_t n_rays, uint32_t rot, uint16_t *chains,
+                 uint8_t *orders, uint32_t cap, uint32_t *n_out);
 
 /* WorldSolve::solve_batch (early_world.rs:38-52; omg-gpu/src/lib.rs:510-609):
  * records[si * n_chains + ci] for every (source, chain).  surfaces: the

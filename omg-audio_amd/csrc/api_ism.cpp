@@ -54,11 +54,45 @@ extern "C" int omg_surface_table(omg_scene* s, omg_surface* out, uint32_t cap, u
     return OMG_OK;
 }
 
-extern "C" int omg_discover(omg_scene* s, const float listener[3], uint32_t n_rays, uint32_t rot,
+// SurfaceTable::append_box (pt_mesh.rs:87-106): the six faces of an overlay
+// box as rect-bounded reflectors with the box's material, in the order
+// x min, x max, y min, y max, z min, z max (the discovery's face ids).
+extern "C" int omg_surface_append_box(const float mn[3], const float mx[3], const omg_material* m,
+                                      uint32_t nbands, omg_surface* out6)
+{
+    if (!mn || !mx || !m || !out6 || nbands < 1 || nbands > OMG_MAX_BANDS)
+        return fail(OMG_ERR_INVALID, "bad argument");
+    int k = 0;
+    for (int axis = 0; axis < 3; axis++) {
+        for (int side = 0; side < 2; side++) {
+            omg_surface& su = out6[k++];
+            std::memset(&su, 0, sizeof su);
+            float n[3] = {0.0f, 0.0f, 0.0f};
+            n[axis] = side == 0 ? -1.0f : 1.0f;
+            const float* corner = side == 0 ? mn : mx;
+            for (int a = 0; a < 3; a++) {
+                su.n[a] = n[a];
+                su.rmin[a] = a == axis ? corner[axis] : mn[a];
+                su.rmax[a] = a == axis ? corner[axis] : mx[a];
+            }
+            su.d = dot(mk(n[0], n[1], n[2]), mk(corner[0], corner[1], corner[2]));
+            for (uint32_t b = 0; b < nbands; b++) {
+                su.refl[b] = sqrtf(fmaxf(1.0f - m->absorption[b], 0.0f));
+                su.trans[b] = m->transmission[b];
+            }
+            su.has_rect = 1;
+        }
+    }
+    return OMG_OK;
+}
+
+extern "C" int omg_discover(omg_scene* s, const float* boxes, uint32_t n_boxes, uint16_t base,
+                            const float listener[3], uint32_t n_rays, uint32_t rot,
                             uint16_t* chains, uint8_t* orders, uint32_t cap, uint32_t* n_out)
 {
-    if (!s || !listener || !n_out || (cap && (!chains || !orders)))
+    if (!s || !listener || !n_out || (cap && (!chains || !orders)) || (n_boxes && !boxes))
         return fail(OMG_ERR_INVALID, "bad argument");
+    if (n_boxes > OMG_MAX_OVERLAYS) return fail(OMG_ERR_CAP, "more than 64 overlay boxes");
     *n_out = 0;
     if (n_rays == 0) return OMG_OK;
     omg_ctx* c = s->ctx;
@@ -76,6 +110,7 @@ extern "C" int omg_discover(omg_scene* s, const float listener[3], uint32_t n_ra
         jitter[i] = (float)(v >> 40) / (float)(1ull << 24);
     }
     int rc = upload(c->aux0, jitter.data(), n_rays * sizeof(float), c->stream);
+    if (!rc && n_boxes) rc = upload(c->aux2, boxes, (size_t)n_boxes * 6 * sizeof(float), c->stream);
     if (rc) return rc;
     OMG_HIP_CHECK(c->aux1.ensure((size_t)n_rays * 8));
     DiscoverArgs A{};
@@ -86,6 +121,9 @@ extern "C" int omg_discover(omg_scene* s, const float listener[3], uint32_t n_ra
     A.rot_phase = (float)rot * 0.61803398875f * kTau;
     A.jitter = c->aux0.as<float>();
     A.n_rays = n_rays;
+    A.boxes = n_boxes ? c->aux2.as<float>() : nullptr;
+    A.n_boxes = n_boxes;
+    A.base = base;
     A.out = c->aux1.as<uint32_t>();
     OMG_HIP_CHECK(launch_discover(A, c->stream));
     std::vector<uint32_t> raw((size_t)n_rays * 2);

@@ -35,6 +35,9 @@ struct DiscoverArgs {
     float rot_phase;     // rot * 0.61803398875 * TAU in f32
     const float* jitter; // [n_rays] xorshift jitter in [0, 1)
     uint32_t n_rays;
+    const float* boxes;  // [n_boxes][6] overlay boxes (min, max) that reflect too
+    uint32_t n_boxes;
+    uint32_t base;       // surface id of the first box's first face
     uint32_t* out;       // [n_rays][2]: chain0 | chain1 << 16, chain2 | levels << 16
 };
 

@@ -1,7 +1,8 @@
 // Image sources on the BVH (crates/omg-core/src/pt_mesh.rs) for gfx950.
 //
 // k_discover: one lane per listener fan ray (mesh_chains, pt_mesh.rs:375-457):
-//   up to three specular bounces, recording the surface-id chain prefix.
+//   up to three specular bounces off the mesh and the overlay boxes,
+//   recording the surface-id chain prefix.
 // k_solve: one lane per (source, chain) (mesh_record, pt_mesh.rs:185-324):
 //   mirror the source across the chain planes, validate every bounce point
 //   by raycast, pay the transmission of every crossing of every leg.
@@ -29,13 +30,62 @@ __global__ __launch_bounds__(128) void k_discover(DiscoverArgs A)
     for (int k = 0; k < 3; k++) {
         float t;
         uint32_t tri, fl;
-        if (!mesh_raycast<false>(A.g, pos, dir, t, tri, fl, tc)) break;
+        bool has = mesh_raycast<false>(A.g, pos, dir, t, tri, fl, tc);
+        uint32_t sid = 0;
+        f3 nn = mk(0.0f, 0.0f, 1.0f);
+        if (has) {
+            sid = A.tri_surface[tri];
+            float4 ti = *reinterpret_cast<const float4*>(A.g.tris + tri);
+            nn = mk(ti.x, ti.y, ti.z);
+        } else {
+            t = FLT_MAX;
+        }
+        // overlay boxes: slab entry from 1e-4, nearest wins (pt_mesh.rs:391-425)
+        for (uint32_t bi = 0; bi < A.n_boxes; bi++) {
+            const float* mn = A.boxes + 6 * bi;
+            const float* mx = mn + 3;
+            float t0 = 1e-4f, t1 = t;
+            int axis = 3;
+            for (int a = 0; a < 3; a++) {
+                const float da = get(dir, a), pa = get(pos, a);
+                if (fabsf(da) < 1e-9f) {
+                    if (pa < mn[a] || pa > mx[a]) {
+                        t0 = FLT_MAX;
+                        break;
+                    }
+                } else {
+                    float ta = (mn[a] - pa) / da, tb = (mx[a] - pa) / da;
+                    if (ta > tb) {
+                        const float sw = ta;
+                        ta = tb;
+                        tb = sw;
+                    }
+                    if (ta > t0) {
+                        t0 = ta;
+                        axis = a;
+                    }
+                    t1 = fminf(t1, tb);
+                    if (t0 > t1) {
+                        t0 = FLT_MAX;
+                        break;
+                    }
+                }
+            }
+            if (t0 < t && axis < 3) {
+                t = t0;
+                const float da = get(dir, axis);
+                const uint32_t face = (uint32_t)axis * 2u + (da > 0.0f ? 0u : 1u);
+                sid = (A.base + bi * 6u + face) & 0xFFFFu;
+                const float sgn = da > 0.0f ? -1.0f : 1.0f;
+                nn = mk(axis == 0 ? sgn : 0.0f, axis == 1 ? sgn : 0.0f, axis == 2 ? sgn : 0.0f);
+                has = true;
+            }
+        }
+        if (!has) break;
         if (t <= 1e-4f || t > 200.0f) break;
         pos = add(pos, mul(dir, t));
-        chain[k] = A.tri_surface[tri];
+        chain[k] = sid;
         levels = (uint32_t)k + 1;
-        float4 ti = *reinterpret_cast<const float4*>(A.g.tris + tri);
-        f3 nn = mk(ti.x, ti.y, ti.z);
         f3 n = dot(nn, dir) > 0.0f ? mul(nn, -1.0f) : nn;
         dir = sub(dir, mul(n, 2.0f * dot(dir, n)));
         pos = add(pos, mul(n, 1e-4f));

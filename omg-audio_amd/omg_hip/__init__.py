@@ -434,9 +434,9 @@ def ism_box(size, walls, src, lis, max_order=3, nbands=3):
 
 
 from .dome import DomeProbe, dome_params, dome_sample, dome_sample_async, dome_status  # noqa: E402
-from .early import GpuWorldDiscovery, GpuWorldSolve, surface_table  # noqa: E402
+from .early import GpuWorldDiscovery, GpuWorldSolve, append_box, surface_table  # noqa: E402
 
-__all__ += ["DomeProbe", "dome_params", "dome_sample", "dome_sample_async", "dome_status", "GpuWorldDiscovery", "GpuWorldSolve",
+__all__ += ["DomeProbe", "dome_params", "dome_sample", "dome_sample_async", "dome_status", "GpuWorldDiscovery", "GpuWorldSolve", "append_box",
             "surface_table"]
 from .batch import Batch, QuerySpec  # noqa: E402
 

@@ -283,7 +283,9 @@ SIGNATURES = [
     ("omg_dome_probe_destroy", None, [_vp]),
     ("omg_ism_box", _i, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, C.POINTER(_u32)]),
     ("omg_surface_table", _i, [_vp, _vp, _u32, C.POINTER(_u32)]),
-    ("omg_discover", _i, [_vp, _vp, _u32, _u32, _vp, _vp, _u32, C.POINTER(_u32)]),
+    ("omg_surface_append_box", _i, [_vp, _vp, _vp, _u32, _vp]),
+    ("omg_discover", _i, [_vp, _vp, _u32, C.c_uint16, _vp, _u32, _u32, _vp, _vp, _u32,
+                          C.POINTER(_u32)]),
     ("omg_solve_batch", _i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp,
                              _vp]),
     ("omg_set_route_rays", _i, [_vp, C.c_uint64]),

@@ -22,6 +22,22 @@ def surface_table(scene, cap=1 << 16):
     return arr, n.value
 
 
+def append_box(table, n_surf, mn, mx, material, nbands=3):
+    """SurfaceTable::append_box (pt_mesh.rs:87-106): writes the box's six faces
+    at table[n_surf:n_surf + 6]; returns the new count (the faces' base id is
+    n_surf).  material: a MaterialDef."""
+    from . import _materials
+    if n_surf + 6 > len(table):
+        raise ValueError("surface table too small")
+    mat = _materials([material])
+    check(lib().omg_surface_append_box(np.asarray(mn, np.float32).ctypes.data_as(C.c_void_p),
+                                       np.asarray(mx, np.float32).ctypes.data_as(C.c_void_p),
+                                       C.cast(mat, C.c_void_p), nbands,
+                                       C.cast(C.byref(table, n_surf * C.sizeof(Surface)),
+                                              C.c_void_p)))
+    return n_surf + 6
+
+
 class GpuWorldDiscovery:
     """WorldDiscovery::discover (early_world.rs:24-26): rotating golden fan."""
 
@@ -29,13 +45,17 @@ class GpuWorldDiscovery:
         self.scene = scene
         self.n_rays = n_rays
 
-    def discover(self, listener, rot, cap=1 << 16):
+    def discover(self, listener, rot, cap=1 << 16, boxes=(), base=0):
+        """Chains found by the fan from `listener`; boxes: overlay (min, max)
+        pairs whose faces reflect too (ids base + box * 6 + face)."""
         ch = np.zeros((cap, 3), np.uint16)
         od = np.zeros(cap, np.uint8)
         n = C.c_uint32()
         lis = np.asarray(listener, np.float32)
-        check(lib().omg_discover(self.scene.h, _p(lis), self.n_rays, rot, _p(ch), _p(od), cap,
-                                 C.byref(n)))
+        bx = (np.ascontiguousarray([list(mn) + list(mx) for mn, mx in boxes], np.float32)
+              if len(boxes) else None)
+        check(lib().omg_discover(self.scene.h, _p(bx), len(boxes), base, _p(lis), self.n_rays, rot,
+                                 _p(ch), _p(od), cap, C.byref(n)))
         return ch[:n.value].copy(), od[:n.value].copy()
 
 

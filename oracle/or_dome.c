@@ -496,8 +496,9 @@ static int set_insert(u64set *s, uint64_t k)
     return 1;
 }
 
-int or_mesh_chains(const or_mesh *m, const float lisf[3], uint32_t n_rays, uint32_t rot,
-                   int use_project_trig, uint16_t *chains, uint8_t *orders, int cap)
+int or_mesh_chains(const or_mesh *m, const float *boxes, int n_boxes, uint16_t base,
+                   const float lisf[3], uint32_t n_rays, uint32_t rot, int use_project_trig,
+                   uint16_t *chains, uint8_t *orders, int cap)
 {
     u64set seen = {0, 0, 0};
     float ga = OR_PI * (3.0f - sqrtf(5.0f));
@@ -523,10 +524,52 @@ int or_mesh_chains(const or_mesh *m, const float lisf[3], uint32_t n_rays, uint3
             uint32_t tri;
             vstore(pf, pos);
             vstore(df, dir);
-            if (!or_mesh_raycast(m, pf, df, &t, &tri, NULL, NULL))
+            int has = or_mesh_raycast(m, pf, df, &t, &tri, NULL, NULL);
+            if (!has) t = FLT_MAX;
+            uint16_t sid = has ? m->tri_surface[tri] : 0;
+            v3 normal = has ? m->normals[tri] : V(0.0f, 0.0f, 1.0f);
+            /* overlay boxes: slab-entry test from 1e-4, nearest wins (pt_mesh.rs:391-425) */
+            for (int bi = 0; bi < n_boxes; bi++) {
+                const float *mn = boxes + 6 * bi, *mx = boxes + 6 * bi + 3;
+                float t0 = 1e-4f, t1 = t;
+                int axis = 3;
+                for (int a = 0; a < 3; a++) {
+                    float da = df[a];
+                    if (fabsf(da) < 1e-9f) {
+                        if (pf[a] < mn[a] || pf[a] > mx[a]) {
+                            t0 = FLT_MAX;
+                            break;
+                        }
+                    } else {
+                        float ta = (mn[a] - pf[a]) / da, tb = (mx[a] - pf[a]) / da;
+                        if (ta > tb) {
+                            float sw = ta;
+                            ta = tb;
+                            tb = sw;
+                        }
+                        if (ta > t0) {
+                            t0 = ta;
+                            axis = a;
+                        }
+                        t1 = fminf(t1, tb);
+                        if (t0 > t1) {
+                            t0 = FLT_MAX;
+                            break;
+                        }
+                    }
+                }
+                if (t0 < t && axis < 3) {
+                    t = t0;
+                    int face = axis * 2 + (df[axis] > 0.0f ? 0 : 1);
+                    sid = (uint16_t)(base + (uint16_t)(bi * 6 + face));
+                    float nv[3] = {0.0f, 0.0f, 0.0f};
+                    nv[axis] = df[axis] > 0.0f ? -1.0f : 1.0f;
+                    normal = V(nv[0], nv[1], nv[2]);
+                    has = 1;
+                }
+            }
+            if (!has)
                 break;
-            uint16_t sid = m->tri_surface[tri];
-            v3 normal = m->normals[tri];
             if (t <= 1e-4f || t > 200.0f)
                 break;
             pos = vadd(pos, vmul(dir, t));

@@ -265,10 +265,13 @@ typedef struct {
 int or_mesh_record(const or_mesh *m, const or_surface *table, int n_surf, const uint16_t *chain,
                    int order, const float src[3], const float lis[3], const or_aabb *extras,
                    int n_extras, float air0, float air1, float air2, or_record *rec);
-/* mesh_chains (pt_mesh.rs:375-457) without overlay boxes: chains as
- * [3] u16 + order, deduplicated in first-seen order.  Returns count. */
-int or_mesh_chains(const or_mesh *m, const float lis[3], uint32_t n_rays, uint32_t rot,
-                   int use_project_trig, uint16_t *chains, uint8_t *orders, int cap);
+/* mesh_chains (pt_mesh.rs:375-457): the rotating fan over the mesh and the
+ * overlay boxes [n_boxes][6] (min, max; face ids base + box*6 + face):
+ * chains as [3] u16 + order, deduplicated in first-seen order.  Returns
+ * the count. */
+int or_mesh_chains(const or_mesh *m, const float *boxes, int n_boxes, uint16_t base,
+                   const float lis[3], uint32_t n_rays, uint32_t rot, int use_project_trig,
+                   uint16_t *chains, uint8_t *orders, int cap);
 
 /* ---------------------------------------------------------------- knife-edge paths */
 /* DiffractionEdge (mesh.rs:626-634) */

@@ -135,7 +135,8 @@ def lib():
         L.or_surface_append_box.argtypes = [vp, vp, vp, vp, u32]
         L.or_mesh_record.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, f32, f32,
                                      f32, vp]
-        L.or_mesh_chains.argtypes = [vp, vp, u32, u32, C.c_int, vp, vp, C.c_int]
+        L.or_mesh_chains.argtypes = [vp, vp, C.c_int, C.c_uint16, vp, u32, u32, C.c_int, vp, vp,
+                                     C.c_int]
         L.or_paths_edges.restype = C.c_int
         L.or_paths_edges.argtypes = [vp, f32, u32, vp, C.c_int]
         L.or_paths_transmission.argtypes = [vp, vp, vp, vp]
@@ -602,9 +603,12 @@ def mesh_record(mesh, table, n_surf, chain, order, src, lis, extras=(), air=None
                 gains=np.array(rec.gains[:mesh.nbands], np.float32))
 
 
-def mesh_chains(mesh, lis, n_rays, rot, project_trig=True, cap=1 << 16):
+def mesh_chains(mesh, lis, n_rays, rot, project_trig=True, cap=1 << 16, boxes=(), base=0):
+    """mesh_chains (pt_mesh.rs:375-457); boxes: overlay (min, max) pairs whose
+    faces reflect too, ids base + box * 6 + face."""
     ch = np.zeros((cap, 3), np.uint16)
     od = np.zeros(cap, np.uint8)
-    n = lib().or_mesh_chains(mesh.ptr, _p(_f32(lis)), n_rays, rot, int(project_trig), _p(ch),
-                             _p(od), cap)
+    bx = _f32([list(mn) + list(mx) for mn, mx in boxes]) if len(boxes) else None
+    n = lib().or_mesh_chains(mesh.ptr, _p(bx), len(boxes), base, _p(_f32(lis)), n_rays, rot,
+                             int(project_trig), _p(ch), _p(od), cap)
     return ch[:n].copy(), od[:n].copy()

@@ -166,3 +166,69 @@ def test_discovery_and_solve_parity(ctx, oracle):
     assert direct.size == 1
     true_delay = np.float32(np.linalg.norm(np.subtract(src, lis)) / 343.0)
     assert abs(rec["delay"][direct[0]] - true_delay) < 1e-5
+
+
+def _solve_vs_oracle(solver, om, otab, on, srcs, chains, orders, lis, extras=()):
+    rec = solver.solve_batch(srcs, chains, orders, lis, extras=extras)
+    n = len(orders)
+    n_valid = 0
+    for si, s in enumerate(srcs):
+        for ci in range(n):
+            r = O.mesh_record(om, otab, on, chains[ci][:orders[ci]], int(orders[ci]), s, lis,
+                              extras=extras)
+            k = si * n + ci
+            assert rec["valid"][k] == (r is not None), f"source {si} chain {ci}: validity differs"
+            if r is None:
+                continue
+            n_valid += 1
+            assert rec["delay"][k] == r["delay"] and np.array_equal(rec["dir"][k], r["dir"])
+            np.testing.assert_allclose(rec["gains"][k], r["gains"], rtol=2e-6)
+    return rec, n_valid
+
+
+def test_overlay_faces_boxes_and_extras_match_the_oracle(ctx, oracle):
+    """Furniture through the C ABI: the table-top room of pt_mesh_gate.rs:
+    180-240 - discovery over the box (omg_discover with boxes) gives the
+    oracle's chains and finds the top face; the solve of chains that reflect
+    off the box faces (omg_surface_append_box rows, rect-validated) and of
+    legs crossing blockers (extras, Aabb::clip) gives the oracle's records,
+    with the exact table-top delay and the off-rect refusal."""
+    from test_ism_overlays_cpu import OPAQUE, golden_box, oracle_table_with_box
+    mb = golden_box((6.0, 4.0, 3.0))
+    scene = H.Scene.from_builder(ctx, mb)
+    om = oracle_mesh_from_builder(mb)
+    mn, mx = (2.0, 1.5, 0.0), (3.0, 2.5, 0.8)
+    tab, n = H.surface_table(scene)
+    otab, on = oracle_table_with_box(om, mn, mx, H.WOOD_PANEL)
+    assert n == on
+    assert H.append_box(tab, n, mn, mx, H.WOOD_PANEL) == n + 6
+    top = n + 5
+    src, lis = (1.0, 2.0, 1.5), (4.0, 2.0, 1.6)
+    disc = H.GpuWorldDiscovery(scene, n_rays=4096)
+    found = set()
+    for rot in range(4):
+        gc, go = disc.discover(lis, rot, boxes=[(mn, mx)], base=n)
+        oc, oo = O.mesh_chains(om, lis, 4096, rot, boxes=[(mn, mx)], base=n)
+        assert np.array_equal(gc, oc) and np.array_equal(go, oo), f"rot {rot}: chains differ"
+        found |= {(tuple(int(x) for x in c[:o]), int(o)) for c, o in zip(gc, go)}
+    assert any(top in c for c, _ in found), "discovery must bounce off the table top"
+    _say(f"discovery over the box: {len(found)} chains identical, top face found")
+    solver = H.GpuWorldSolve(scene, tab, n + 6)
+    lst = sorted(found)
+    chains = np.array([list(c) + [0xFFFF] * (3 - o) for c, o in [((), 0)] + lst], np.uint16)
+    orders = np.array([0] + [o for _, o in lst], np.uint8)
+    srcs = [src, (5.0, 3.0, 1.2)]
+    blockers = [((2.6, 0.5, 0.0), (2.8, 3.5, 1.2), OPAQUE), ((4.5, 0.2, 1.0), (5.5, 1.0, 2.8),
+                                                              (0.5, 0.25, 0.125))]
+    for extras in ((), blockers):
+        rec, nv = _solve_vs_oracle(solver, om, otab, n + 6, srcs, chains, orders, lis, extras)
+        _say(f"solve with {len(extras)} blockers: {nv} records match the oracle")
+    # the table-top chain alone: exact mirror delay, refused off the rect
+    one = np.array([[top, 0xFFFF, 0xFFFF]], np.uint16)
+    rec = solver.solve_batch([src], one, np.array([1], np.uint8), lis)
+    img = np.array([1.0, 2.0, 2.0 * 0.8 - 1.5])
+    assert rec["valid"][0]
+    assert abs(float(rec["delay"][0]) - np.linalg.norm(img - np.array(lis)) / 343.0) < 1e-5
+    off = solver.solve_batch([src], one, np.array([1], np.uint8), (1.2, 0.3, 1.6))
+    assert not off["valid"][0]
+    scene.close()
