@@ -107,6 +107,7 @@ def lib():
         L.or_mesh_raycast.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.or_mesh_segment_hits.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, vp]
         L.or_mesh_tri_normal.argtypes = [vp, u32, vp]
+        L.or_raycast_hit.restype = C.c_int
         L.or_raycast_hit.argtypes = [vp, vp, vp, vp, vp]
         L.or_nbins.restype = u32
         L.or_nbins.argtypes = [f32, f32]
@@ -389,6 +390,18 @@ def trace_params(n_rays, nbands=3, max_bounces=64, rng_mode=RNG_COUNTER, seed=0,
     for i, v in enumerate(e):
         p.energy[i] = float(v)
     return p
+
+
+def raycast_hit(geom, p, d):
+    """AcousticGeometry::raycast_hit of a geometry (Shoebox, Mesh, WithPanels;
+    scene.rs:41-138): dict(t, normal, panel (index or None), tri, flags) or
+    None on a miss."""
+    h = Hit()
+    if not lib().or_raycast_hit(C.byref(geom), _p(_f32(p)), _p(_f32(d)), C.byref(h), None):
+        return None
+    panel = -h.material_ref - 1 if h.material_ref < 0 else None
+    return dict(t=np.float32(h.t), normal=np.array(h.normal[:], np.float32), panel=panel,
+                tri=h.tri, flags=h.flags)
 
 
 def trace(geom, src, lis, params, rng_state=None, want_rays=False, want_hits=False,

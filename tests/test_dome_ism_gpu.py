@@ -232,3 +232,75 @@ def test_overlay_faces_boxes_and_extras_match_the_oracle(ctx, oracle):
     off = solver.solve_batch([src], one, np.array([1], np.uint8), (1.2, 0.3, 1.6))
     assert not off["valid"][0]
     scene.close()
+
+
+def _probe_restated(passes, nb):
+    """DomeProbe::sample's EMA and bin directions (dome.rs:460-477) restated
+    over the oracle's per-pass bin sums: alpha 1 on the first call, 0.35
+    after; dir = normalize(dsum) (times the reciprocal length), or, for a
+    silent bin, its sector centre (zenith (0, 0, 1))."""
+    f = np.float32
+    ema = np.zeros((9, nb), np.float32)
+    outs = []
+    for k, (e, dsum) in enumerate(passes):
+        alpha = f(1.0) if k == 0 else f(0.35)
+        ema = (ema + alpha * (e - ema)).astype(np.float32)
+        out = np.zeros((9, 3 + nb), np.float32)
+        for b in range(9):
+            x, y, z = (f(v) for v in dsum[b])
+            ln = np.sqrt(f(f(f(x * x) + f(y * y)) + f(z * z)))
+            if ln > f(1e-6):
+                inv = f(1.0) / ln
+                out[b, :3] = [x * inv, y * inv, z * inv]
+            elif b == 8:
+                out[b, :3] = [0.0, 0.0, 1.0]
+            else:
+                az = f(f(f(b) + f(0.5)) * f(2.0 * np.pi) / f(8.0)) - f(np.pi)
+                out[b, :3] = [np.cos(az), np.sin(az), 0.0]
+            out[b, 3:] = ema[b]
+        outs.append(out)
+    return outs
+
+
+@pytest.mark.parametrize("n_rays", [384, 256, 160])
+def test_dome_probe_drop_in(ctx, oracle, n_rays):
+    """omg_dome_probe_sample, the DomeProbe::sample drop-in (dome.rs:425-478):
+    three calls from three eyes (the EMA primes on the first, then blends at
+    0.35), at the full fan and the 256- and 160-ray tiers (even stride
+    j * 384 / n, dome.rs:431-434), in a sealed room whose walls pass
+    nothing and a house with a door and a window: every output row equals
+    the restatement over the oracle's bin sums; in the sealed room every
+    bin is silent and carries its placeholder direction."""
+    mb, glass, leaf = house()
+    scene = H.Scene.from_builder(ctx, mb)
+    om = oracle_mesh_from_builder(mb)
+    params = H.dome_params(3, n_fan=384, n_rays=n_rays)
+    eyes = [(5.0, 4.0, 1.6), (8.5, 4.0, 1.6), (12.0, 4.0, 1.6)]
+    probe = H.DomeProbe(scene, params)
+    got = [probe.sample(e, [glass, leaf] if k == 1 else [glass]) for k, e in enumerate(eyes)]
+    probe.close()
+    passes = []
+    for k, e in enumerate(eyes):
+        o = O.dome_sample(om, e, n_fan=384, n_rays=n_rays, panels=[glass, leaf] if k == 1 else [glass])
+        passes.append((o["e"], o["dsum"]))
+    want = _probe_restated(passes, 3)
+    for k in range(3):
+        assert np.array_equal(got[k][:, 3:], want[k][:, 3:]), f"call {k}: EMA energies differ"
+        np.testing.assert_allclose(got[k][:, :3], want[k][:, :3], rtol=0, atol=2e-7,
+                                   err_msg=f"call {k}: bin directions differ")
+    scene.close()
+    # sealed room, walls that absorb half and pass nothing: every bounce path
+    # dies inside and every straight line pays zero, so all nine bins are silent
+    sb = scenes.MeshBuilder()
+    m = sb.material(H.MaterialDef([0.5, 0.5, 0.5], 0.0, [0.0, 0.0, 0.0]))
+    sb.abox((0.0, 0.0, 0.0), (6.0, 4.0, 3.0), m)
+    sealed = H.Scene.from_builder(ctx, sb)
+    probe = H.DomeProbe(sealed, params)
+    out = [probe.sample((3.0, 2.0, 1.5)) for _ in range(2)]
+    probe.close()
+    sealed.close()
+    want = _probe_restated([(np.zeros((9, 3), np.float32), np.zeros((9, 3), np.float32))] * 2, 3)
+    for k in range(2):
+        assert np.all(out[k][:, 3:] == 0.0)
+        np.testing.assert_allclose(out[k][:, :3], want[k][:, :3], rtol=0, atol=2e-7)
+    _say(f"dome probe, {n_rays} rays: three calls and a sealed room match the restatement")
