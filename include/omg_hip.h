@@ -272,6 +272,24 @@ int omg_trace_world_async(omg_scene *scene, const omg_trace_params *params,
                           float *d_bins, float *d_dirs, omg_ray_record *d_records,
                           void *stream);
 
+/* Allocation-free variant of omg_trace_world_async, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments, kernels and results; the call allocates
+ * nothing: the job's material and overlay tables and the 16-lane hand-off
+ * queue live in a fixed device array (about 24 MiB per GPU, held until the
+ * process ends) instead of the context's buffers.  Limits: at most 262144
+ * rays (rays per source x sources) and 4096 materials (scene materials +
+ * overlays) per call, refused with OMG_ERR_CAP above.  The tables are
+ * copied (the call then waits for `stream`) only when they differ from the
+ * last call's.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other contexts) race. */
+int omg_trace_world_async_noalloc(omg_scene *scene, const omg_trace_params *params,
+                                  const float *d_sources, uint32_t n_src,
+                                  const float *d_listeners, uint32_t n_lis,
+                                  const omg_overlay *overlays, uint32_t n_ovl, float *d_bins,
+                                  float *d_dirs, omg_ray_record *d_records, void *stream);
+
 /* ------------------------------------------------------------ batched persistent launch */
 
 /* One query of a batch: the arguments of one omg_trace_world call

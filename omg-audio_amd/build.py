@@ -14,7 +14,7 @@ OUT = os.path.join(HERE, "libomg_hip.so")
 SOURCES = ["trace_kernels.hip", "dome_kernels.hip", "ism_kernels.hip", "scene_build.cpp",
            "api_trace.cpp", "api_dome.cpp", "api_ism.cpp", "api_batch.cpp", "api_serve.cpp",
            "late_kernels.hip", "api_late.cpp", "api_wire.cpp", "build_kernels.hip", "api_build.cpp",
-           "paths_kernels.hip", "api_paths.cpp"]
+           "paths_kernels.hip", "api_paths.cpp", "api_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
          "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

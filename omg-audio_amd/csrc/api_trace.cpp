@@ -48,6 +48,11 @@ extern "C" int omg_init(int device, uint32_t layout_version, omg_ctx** out)
         delete c;
         return fail(OMG_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    if (int rc = noalloc_setup(device)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return OMG_OK;
 }

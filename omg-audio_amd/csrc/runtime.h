@@ -159,6 +159,9 @@ void ctx_release(omg_ctx* c);
 void make_fan(uint32_t n_fan, std::vector<float>& fan, std::vector<uint32_t>& bins);
 // waits for in-flight *_async work before shared staging is overwritten
 int quiesce(omg_ctx* c);
+// Resolves `device`'s copy of the allocation-free launch's scratch array
+// (omg_trace_world_async_noalloc); called by omg_init with the device current.
+int noalloc_setup(int device);
 // derives the scene's 16-wide nodes from its resident 4-wide nodes on
 // the device (stream-ordered; waits for the stream at the end)
 int derive16(omg_scene* s, hipStream_t st);

@@ -326,6 +326,19 @@ def trace_world_async(scene, params, d_sources, n_src, d_listeners, n_lis, d_bin
                                       C.c_void_p(stream) if stream else None))
 
 
+def trace_world_async_noalloc(scene, params, d_sources, n_src, d_listeners, n_lis, d_bins,
+                              d_dirs, stream=None, overlays=(), d_records=None):
+    """trace_world_async with no allocation in the call (a fixed device
+    array holds its scratch): at most 262144 rays and 4096 materials."""
+    ov = _overlays(list(overlays))
+    check(lib().omg_trace_world_async_noalloc(scene.h, C.byref(params), C.c_void_p(d_sources),
+                                              n_src, C.c_void_p(d_listeners), n_lis,
+                                              C.cast(ov, C.c_void_p), len(overlays),
+                                              C.c_void_p(d_bins), C.c_void_p(d_dirs),
+                                              C.c_void_p(d_records) if d_records else None,
+                                              C.c_void_p(stream) if stream else None))
+
+
 def estimate_reverb(bins, bin_dt):
     """tracer.rs:205-257 -> (rt60[nb], level[nb])."""
     b = _f32(bins)

@@ -242,6 +242,8 @@ SIGNATURES = [
     ("omg_trace_box", _i, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("omg_trace_world_async", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp,
                                    _vp]),
+    ("omg_trace_world_async_noalloc", _i, [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp,
+                                           _vp, _vp]),
     ("omg_batch_create", _i, [_vp, _vp, _u32, _u32, C.POINTER(_vp)]),
     ("omg_batch_launch", _i, [_vp, _vp]),
     ("omg_batch_sync", _i, [_vp]),
