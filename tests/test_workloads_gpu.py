@@ -64,6 +64,7 @@ def _vs_oracle(records, bins, dirs, outs, n_rays, label, dir_band=1, sources=Non
     """records/bins/dirs of the GPU against oracle traces (one per source)."""
     rec = records.reshape(-1, n_rays, 4)
     worst_b = worst_d = 0.0
+    heard = 0
     for k, o in enumerate(outs):
         si = k if sources is None else sources[k]
         diff = np.nonzero(np.any(rec[si] != o["recs"], axis=1))[0]
@@ -71,8 +72,10 @@ def _vs_oracle(records, bins, dirs, outs, n_rays, label, dir_band=1, sources=Non
                                 f"gpu {rec[si][diff[:3]].tolist()} oracle {o['recs'][diff[:3]].tolist()}")
         eb, ed = echogram_rel_errors(bins[si], dirs[si], o["bins64"], o["dirs64"], dir_band)
         worst_b, worst_d = max(worst_b, eb), max(worst_d, ed)
+        heard += int(np.count_nonzero(o["bins64"]))
     _say(f"  {label}: {len(outs)} x {n_rays} rays bit-identical to the oracle; echogram rel err "
-         f"vs f64 bins {worst_b:.2e} dirs {worst_d:.2e}")
+         f"vs f64 bins {worst_b:.2e} dirs {worst_d:.2e} over {heard} non-zero bins")
+    assert heard > 0, f"{label}: no listener hears anything"
     assert worst_b <= ECHO_TOL and worst_d <= ECHO_TOL, (label, worst_b, worst_d)
 
 
