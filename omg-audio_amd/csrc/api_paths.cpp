@@ -23,6 +23,10 @@ struct omg_paths {
     DevBuf d_edges, d_src, d_lis, d_out, d_n, d_work, d_cnt, d_trav, d_plan;
     bool counting = false;
     omg_counters last{};
+    // an async search ran since omg_paths_status last read the overflow
+    // word, and an overflow it flagged that a blocking call has since cleared
+    bool async_unread = false;
+    bool pending_overflow = false;
 };
 
 namespace {
@@ -33,6 +37,20 @@ int32_t total_key(float f)
     int32_t k;
     std::memcpy(&k, &f, 4);
     return k ^ (int32_t)((uint32_t)(k >> 31) >> 1);
+}
+
+// Before a blocking call clears the counter words: an overflow flagged by
+// an earlier async search stays pending for omg_paths_status.
+int keep_async_overflow(omg_paths* p, omg_ctx* c)
+{
+    if (!p->async_unread) return OMG_OK;
+    uint32_t flag = 0;
+    OMG_HIP_CHECK(hipMemcpyAsync(&flag, p->d_cnt.as<uint32_t>() + 1, sizeof flag,
+                                 hipMemcpyDeviceToHost, c->stream));
+    OMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    p->pending_overflow |= flag != 0;
+    p->async_unread = false;
+    return OMG_OK;
 }
 
 int check_budget(const omg_paths* p, const omg_path_budget* b)
@@ -226,6 +244,7 @@ extern "C" int omg_paths_transmission(omg_paths* p, const float* a, const float*
     omg_ctx* c = p->scene->ctx;
     OMG_HIP_CHECK(hipSetDevice(c->device));
     int rc = quiesce(c);
+    if (!rc) rc = keep_async_overflow(p, c);
     if (rc) return rc;
     rc = upload(p->d_src, a, (size_t)n * 3 * sizeof(float), c->stream);
     if (!rc) rc = upload(p->d_lis, b, (size_t)n * 3 * sizeof(float), c->stream);
@@ -258,6 +277,7 @@ extern "C" int omg_paths_find(omg_paths* p, const omg_path_budget* b, const floa
     omg_ctx* c = p->scene->ctx;
     OMG_HIP_CHECK(hipSetDevice(c->device));
     rc = quiesce(c);
+    if (!rc) rc = keep_async_overflow(p, c);
     if (rc) return rc;
     const size_t stride = 1 + (size_t)b->max_paths;
     rc = upload(p->d_src, src, (size_t)n * 3 * sizeof(float), c->stream);
@@ -324,6 +344,9 @@ extern "C" int omg_paths_find_async(omg_paths* p, const omg_path_budget* b, cons
     A.work = d_work;
     A.counters = p->d_cnt.as<uint32_t>();
     A.trav = p->d_trav.as<unsigned long long>();
+    // blocking calls on this context wait for it before reusing its buffers
+    c->async_inflight = true;
+    p->async_unread = true;
     OMG_HIP_CHECK(launch_paths(A, false, st));
     return OMG_OK;
 }
@@ -338,7 +361,11 @@ extern "C" int omg_paths_status(omg_paths* p)
     OMG_HIP_CHECK(hipMemcpy(cnt, p->d_cnt.ptr, sizeof cnt, hipMemcpyDeviceToHost));
     OMG_HIP_CHECK(launch_fill_zero(p->d_cnt.as<float>(), 2, c->stream));
     OMG_HIP_CHECK(hipStreamSynchronize(c->stream));
-    return cnt[1] ? fail(OMG_ERR_OVERFLOW, "segment crossing list overflowed") : OMG_OK;
+    c->async_inflight = false;
+    const bool over = cnt[1] != 0 || p->pending_overflow;
+    p->pending_overflow = false;
+    p->async_unread = false;
+    return over ? fail(OMG_ERR_OVERFLOW, "segment crossing list overflowed") : OMG_OK;
 }
 
 extern "C" int omg_paths_set_counting(omg_paths* p, uint32_t on)

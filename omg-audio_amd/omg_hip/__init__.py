@@ -98,7 +98,9 @@ class Context:
 
     def __init__(self, device=0):
         h = C.c_void_p()
-        check(lib().omg_init(device, LAYOUT_VERSION, C.byref(h)))
+        L = lib()
+        _lib.check_single_runtime()     # torch may have been imported since the load
+        check(L.omg_init(device, LAYOUT_VERSION, C.byref(h)))
         self.h = h
         self.device = device
         buf = C.create_string_buffer(128)

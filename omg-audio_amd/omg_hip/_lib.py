@@ -316,13 +316,37 @@ class OmgError(RuntimeError):
         self.code = code
 
 
+def hip_runtimes():
+    """The distinct HIP runtime libraries (libamdhip64) mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            paths = {line.split()[-1] for line in f if "libamdhip64" in line}
+    except OSError:
+        return []
+    return sorted({os.path.realpath(p) for p in paths})
+
+
+def check_single_runtime():
+    """Raises when two HIP runtimes are mapped.  PyTorch wheels carry their
+    own libamdhip64: imported after this library has loaded ROCm's, torch
+    maps a second runtime, whose device pointers and streams mean nothing
+    to this library's.  Import torch before the first use of omg_hip (the
+    library then binds to torch's runtime), or do not import torch."""
+    rts = hip_runtimes()
+    if len(rts) > 1:
+        raise RuntimeError("two HIP runtimes are mapped in this process (" + ", ".join(rts) +
+                           "): import torch before omg_hip loads libomg_hip.so")
+
+
 def lib():
-    """Load libomg_hip.so; raises if it has not been built."""
+    """Load libomg_hip.so; raises if it has not been built, or if loading it
+    would leave two HIP runtimes in the process."""
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built: run omg-audio_amd/build.py")
         L = C.CDLL(LIB_PATH)
+        check_single_runtime()
         for name, res, args in SIGNATURES:
             if not hasattr(L, name):
                 continue

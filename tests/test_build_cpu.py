@@ -30,6 +30,40 @@ def test_library_exports_every_declared_symbol():
     assert not [s for s in syms if s not in bound], "binding lacks declared entry points"
 
 
+_RUNTIME_PROBE = """
+import sys
+sys.path.insert(0, {pkg!r})
+from omg_hip import _lib
+if {torch_first}:
+    import torch
+_lib.lib()
+if not {torch_first}:
+    import torch
+try:
+    _lib.check_single_runtime()
+    print("one", len(_lib.hip_runtimes()))
+except RuntimeError as e:
+    print("refused", e)
+"""
+
+
+@pytest.mark.parametrize("torch_first", [True, False])
+def test_a_second_hip_runtime_is_refused(torch_first):
+    """torch imported first: the library binds to torch's HIP runtime (one
+    mapped); torch imported after the library: two runtimes, refused."""
+    import subprocess
+    import sys
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "omg-audio_amd")
+    out = subprocess.run([sys.executable, "-c", _RUNTIME_PROBE.format(pkg=pkg, torch_first=torch_first)],
+                         capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    last = out.stdout.strip().splitlines()[-1]
+    if torch_first:
+        assert last == "one 1", last
+    else:
+        assert last.startswith("refused") and "import torch before omg_hip" in last, last
+
+
 def _host_build(mb):
     L = _lib.lib()
     soup = mb.soup()

@@ -286,3 +286,34 @@ def test_find_async_matches_find(ctx, city):
     for q in range(n):
         assert np.array_equal(got[q], ref[q]), f"query {q}"
     ap.close()
+
+
+def test_async_overflow_survives_a_blocking_call(ctx):
+    """An async search that overflows, then a blocking search that does not:
+    the blocking call returns OK and waits for the async one, and the next
+    omg_paths_status still reports the async overflow, once."""
+    from omg_hip.paths import PATH_DTYPE
+    a, b = np.float32([0.0, 0.3, 1.0]), np.float32([20.0, 0.3, 1.0])
+    clear_a, clear_b = np.float32([0.0, 5.0, 1.0]), np.float32([20.0, 5.0, 1.0])
+    mb = _wall_stack(40)
+    scene = H.Scene.from_builder(ctx, mb)
+    ap = H.AutoPaths(scene, 0)
+    budget = H.PathBudget()
+    dev = torch.device("cuda", 0)
+    stride = 1 + budget.max_paths
+    d_src = torch.from_numpy(a.reshape(1, 3).copy()).to(dev)
+    d_lis = torch.from_numpy(b.reshape(1, 3).copy()).to(dev)
+    d_out = torch.zeros(stride * PATH_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    d_n = torch.zeros(1, dtype=torch.int32, device=dev)
+    d_work = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    ap.find_async(d_src.data_ptr(), d_lis.data_ptr(), 1, d_out.data_ptr(), d_n.data_ptr(),
+                  d_work.data_ptr(), budget, stream.cuda_stream)
+    paths = ap.find(clear_a, clear_b, budget)
+    assert len(paths) >= 1
+    with pytest.raises(H.OmgError) as e:
+        ap.status()
+    assert e.value.code == H._lib.ERR_OVERFLOW
+    ap.status()
+    ap.close()
+    scene.close()
