@@ -63,31 +63,49 @@ constexpr uint32_t kLatencyItemRays = 16;
 // starting at these sizes (a C5 batch needs a few MB) makes growth rare.
 constexpr size_t kMinArenaBytes = 16u << 20;
 constexpr size_t kMinStagingBytes = 4u << 20;
+// Sizes given to every slot at omg_server_create when the budget leaves
+// each slot at least kMinArenaBytes: a batch of 64 requests of up to 32
+// sources x 16384 rays fits without growing, so a fresh server serves its
+// first requests at its steady-state latency.
+constexpr size_t kPresizeArenaBytes = 64u << 20;
+constexpr size_t kPresizeStagingBytes = 8u << 20;
 
 struct Slot {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;    // dome fans, overlapped with the tracer
     hipEvent_t ev = nullptr, fork = nullptr, join = nullptr;
     DevBuf arena;
+    // pinned staging, mapped into the device's address space: the batch's
+    // copy kernels read host_in and write host_out through *_dev
     void* host_in = nullptr;
+    void* host_in_dev = nullptr;
     size_t host_in_cap = 0;
     void* host_out = nullptr;
+    void* host_out_dev = nullptr;
     size_t host_out_cap = 0;
     std::vector<ReqP> reqs;
     bool busy = false;
     size_t out_off = 0, out_bytes = 0;   // [ctl | outputs | records] copied back
 };
 
-int host_grow(void*& p, size_t& cap, size_t want)
+// Coherent (fine-grained) pinned memory: the device reads and writes it
+// over the bus without caching it, so a copy kernel's reads see the host's
+// latest tables and its writes are visible once the batch's event fires.
+int host_grow(void*& p, void*& dev, size_t& cap, size_t want)
 {
     if (want <= cap) return OMG_OK;
     if (p) (void)hipHostFree(p);
-    p = nullptr;
+    p = dev = nullptr;
     cap = 0;
     size_t n = std::max(want + want / 4 + 4096, kMinStagingBytes);
-    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&p, n, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
         p = nullptr;
         return fail(OMG_ERR_DEVICE, "hipHostMalloc");
+    }
+    if (hipHostGetDevicePointer(&dev, p, 0) != hipSuccess || !dev) {
+        (void)hipHostFree(p);
+        p = dev = nullptr;
+        return fail(OMG_ERR_DEVICE, "hipHostGetDevicePointer");
     }
     cap = n;
     return OMG_OK;
@@ -255,8 +273,8 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         if (S.arena.ensure_exact((size_t)want) != hipSuccess)
             return fail(OMG_ERR_DEVICE, "work arena allocation");
     }
-    rc = host_grow(S.host_in, S.host_in_cap, o_ctl);
-    if (!rc) rc = host_grow(S.host_out, S.host_out_cap, out_end - o_ctl);
+    rc = host_grow(S.host_in, S.host_in_dev, S.host_in_cap, o_ctl);
+    if (!rc) rc = host_grow(S.host_out, S.host_out_dev, S.host_out_cap, out_end - o_ctl);
     if (rc) return rc;
     uint8_t* H = static_cast<uint8_t*>(S.host_in);
     uint8_t* D = S.arena.as<uint8_t>();
@@ -294,7 +312,9 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         }
     }
     hipStream_t st = S.stream;
-    OMG_HIP_CHECK(hipMemcpyAsync(D, H, o_ctl, hipMemcpyHostToDevice, st));
+    // tables in and results out by copy kernels on the batch's stream: a
+    // DMA copy's call can block the scheduler for milliseconds
+    OMG_HIP_CHECK(launch_copy16(D, S.host_in_dev, o_ctl, st));
     OMG_HIP_CHECK(launch_fill_zero(reinterpret_cast<float*>(D + o_ctl), (out_end - o_ctl) / 4, st));
     BatchArgs B{};
     B.jobs = reinterpret_cast<const JobView*>(D + o_jobs);
@@ -340,7 +360,7 @@ int omg_server::launch(uint32_t si, std::vector<ReqP>& batch, uint64_t room)
         OMG_HIP_CHECK(hipStreamWaitEvent(st, S.join, 0));
     }
     if (!S.side) OMG_HIP_CHECK(launch_trace_batch(B, nbm, sh, ctx->n_cu, false, st, max_blocks));
-    OMG_HIP_CHECK(hipMemcpyAsync(S.host_out, D + o_ctl, out_end - o_ctl, hipMemcpyDeviceToHost, st));
+    OMG_HIP_CHECK(launch_copy16(S.host_out_dev, D + o_ctl, out_end - o_ctl, st));
     OMG_HIP_CHECK(hipEventRecord(S.ev, st));
     S.out_off = o_ctl;
     S.out_bytes = out_end - o_ctl;
@@ -543,7 +563,65 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
     if (!rc) rc = upload(s->fan, fan.data(), fan.size() * 4, c->stream);
     if (!rc) rc = upload(s->fan_bin, fan_bin.data(), fan_bin.size() * 4, c->stream);
     if (!rc && s->no_panels.ensure(sizeof(DomePanelDev)) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "alloc");
+    // Start-up work a first batch would otherwise pay for (tens of ms:
+    // device and pinned allocations, which wait for the whole device, the
+    // load of the kernels' code objects, each stream's first hardware
+    // queue): every slot gets its arena and staging at their working size
+    // (as far as the budget allows) and runs one tiny kernel of each
+    // kernel file on each of its streams.
+    const uint64_t share = (cfg->hbm_budget - sb) / cfg->slots;
+    for (Slot& S : s->slots) {
+        if (rc) break;
+        if (share >= kMinArenaBytes &&
+            S.arena.ensure_exact((size_t)std::min<uint64_t>(share, kPresizeArenaBytes)) != hipSuccess)
+            rc = fail(OMG_ERR_DEVICE, "work arena allocation");
+        if (!rc) rc = host_grow(S.host_in, S.host_in_dev, S.host_in_cap, kPresizeStagingBytes);
+        if (!rc) rc = host_grow(S.host_out, S.host_out_dev, S.host_out_cap, kPresizeStagingBytes);
+        for (hipStream_t q : {S.stream, S.side}) {
+            if (rc || !q) continue;
+            float* scratch = S.arena.cap ? S.arena.as<float>() : s->no_panels.as<float>();
+            if (launch_fill_zero(scratch, 1, q) != hipSuccess || launch_dome_prime(scratch, q) != hipSuccess)
+                rc = fail(OMG_ERR_DEVICE, "stream start-up");
+        }
+    }
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = fail(OMG_ERR_DEVICE, "server start-up");
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "fan upload");
+    // then one 16-ray request per slot through the whole batch path (copies,
+    // clear, tracer, dome fans, read-back), results discarded
+    for (uint32_t si = 0; !rc && n_scenes && si < cfg->slots; si++) {
+        auto r = std::make_shared<Req>();
+        r->scene = scenes[0];
+        omg_trace_params& p = r->p;
+        p.n_rays = 16;
+        p.nbands = scenes[0]->nbands;
+        p.max_bounces = 8;
+        p.dir_band = p.nbands > 1 ? 1 : 0;
+        p.seed = 0;
+        p.bin_dt = 0.01f;
+        p.max_time = 0.05f;
+        p.floor_rel = 1e-7f;
+        p.radius = 0.5f;
+        for (uint32_t b = 0; b < OMG_MAX_BANDS; b++) p.energy[b] = 1.0f;
+        r->src.assign(3, 0.0f);
+        r->lis.assign(3, 1.0f);
+        r->n_src = r->n_lis = 1;
+        r->dome = p.nbands >= 2 ? 1u : 0u;
+        r->nbands = p.nbands;
+        rc = validate_params(&p, p.nbands, 1, 1, 0, &r->nbins);
+        if (rc) break;
+        r->footprint = s->footprint(*r);
+        Slot& S = s->slots[si];
+        // within the budget: the room this slot's arena may take
+        const uint64_t others = s->committed() - S.arena.cap;
+        const uint64_t room = cfg->hbm_budget > others ? cfg->hbm_budget - others : 0;
+        if (r->footprint + 4096 > room) continue;
+        std::vector<ReqP> one{r};
+        rc = s->launch(si, one, room);
+        if (!rc && S.side && hipStreamSynchronize(S.side) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "start-up");
+        if (!rc && hipStreamSynchronize(S.stream) != hipSuccess) rc = fail(OMG_ERR_DEVICE, "start-up");
+        S.reqs.clear();
+        S.busy = false;
+    }
     if (rc) {
         for (Slot& S : s->slots) {
             for (hipEvent_t e : {S.ev, S.fork, S.join})
@@ -551,13 +629,18 @@ extern "C" int omg_server_create(omg_ctx* c, const omg_server_config* cfg, omg_s
             for (hipStream_t q : {S.stream, S.side})
                 if (q) (void)hipStreamDestroy(q);
         }
+        for (Slot& S : s->slots) {
+            S.arena.release();
+            for (void* h : {S.host_in, S.host_out})
+                if (h) (void)hipHostFree(h);
+        }
         s->fan.release();
         s->fan_bin.release();
         s->no_panels.release();
         delete s;
         return rc;
     }
-    s->st.committed_bytes = s->st.peak_committed_bytes = sb;
+    s->st.committed_bytes = s->st.peak_committed_bytes = s->committed();
     s->t0 = Clock::now();
     c->live_scenes++;
     s->th = std::thread([s] { s->loop(); });

@@ -47,5 +47,8 @@ struct DomeArgs {
 };
 
 hipError_t launch_dome(const DomeArgs& A, bool count, hipStream_t s);
+// One tiny launch that makes the device load this file's kernels (a code
+// object loads on the first launch of any of its kernels): writes 0 to p[0].
+hipError_t launch_dome_prime(float* p, hipStream_t s);
 
 }  // namespace omg

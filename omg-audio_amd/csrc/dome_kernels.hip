@@ -232,6 +232,17 @@ static hipError_t launch_dome_t(const DomeArgs& A, bool count, hipStream_t s)
     return hipGetLastError();
 }
 
+__global__ void k_dome_prime(float* p)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) p[0] = 0.0f;
+}
+
+hipError_t launch_dome_prime(float* p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dome_prime, dim3(1), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
 hipError_t launch_dome(const DomeArgs& A, bool count, hipStream_t s)
 {
     switch (A.nbands) {

@@ -140,6 +140,8 @@ namespace omg {
 hipError_t launch_trace(const TraceArgs& A, bool mesh, bool count, hipStream_t s);
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s);
 hipError_t launch_fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
+// Copies `bytes` (a multiple of 16, both ends 16-byte aligned) with a kernel on s.
+hipError_t launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t s);
 int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s);
 uint32_t pick_lanes(const omg_ctx* c, uint64_t n_rays);
 // lanes per ray, BVH width, LDS node prefix and stack of a plain launch

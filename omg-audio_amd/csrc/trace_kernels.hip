@@ -771,6 +771,16 @@ __global__ void k_fill_zero_u64(unsigned long long* p, size_t n)
     if (i < n) p[i] = 0ull;
 }
 
+// 16-byte copy, grid-stride: moves the serving loop's batch tables and
+// results between mapped pinned host memory and the device on the batch's
+// own stream (no DMA engine in the request's path)
+__global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------ launchers
 
 constexpr uint32_t kTraceBlock = 256;
@@ -884,6 +894,18 @@ hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s)
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_zero, dim3((uint32_t)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return hipSuccess;
+    if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return hipErrorInvalidValue;
+    size_t n = bytes / 16;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_copy16, dim3((uint32_t)blocks), dim3(256), 0, s, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), n);
     return hipGetLastError();
 }
 
