@@ -508,7 +508,7 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
 // entries), and inside a half the nearer slot first (again left on ties).
 // The halves' boxes are not stored: a half's box is the union of its
 // slots' boxes, whose slab entry follows from the slots' per-axis entries
-// (entry_key, half_key).  Culling is the binary walk's (`t_enter >= best` at pop,
+// (half_entry, half_key).  Culling is the binary walk's (`t_enter >= best` at pop,
 // the current best as tmax of the box test), so the 4-wide walk visits the
 // binary walk's leaves in the binary walk's order with the same best
 // distance at every step, and its nearest hit - triangle, distance and the
@@ -536,7 +536,7 @@ __device__ __forceinline__ Cand4 csel(bool c, const Cand4& a, const Cand4& b)
 
 // Slab test of one slot box that also returns the per-axis entries (the
 // lesser of the two plane distances, as the slab test takes them).  For a
-// finite 1/d (the wide walks hand other rays to the binary walk) the plane
+// finite 1/d (the quad and 16-lane walks hand other rays to the binary walk) the plane
 // distances are numbers or infinities, never NaN, so the swap of the
 // reference's test is a min / max and the result is ray_aabb's.
 __device__ __forceinline__ bool slab_entries(const float4& lo, const float4& hi, f3 o, f3 inv,
@@ -556,17 +556,62 @@ __device__ __forceinline__ bool slab_entries(const float4& lo, const float4& hi,
     return !(t0 > t1);
 }
 
-// Entry keys.  A slot's per-axis entry e enters the group computations as
-// the key max(0, e) in the bits of the float (a NaN or -0 gives 0):
-// non-negative floats and +inf order as their bit patterns, so the entry
-// of a union box is the integer minimum of its parts' keys per axis and
-// a box's entry distance the integer maximum over its axes.  This holds
-// for a finite 1/d, where every entry is a number (an overflow an
-// infinity) and rounding is monotone: the union's per-axis entry is the
-// least of its parts', and max(0, .) commutes with the minimum.  A ray
-// with an infinite 1/d component (a zero direction component) can meet
-// NaN entries, for which neither holds; the wide walks hand such rays to
-// the binary walk (mesh_raycast), which is the reference walk itself.
+// Slab test with per-axis entries for any 1/d, in the reference's form: a
+// plane distance pair is swapped when the first is the greater, so for a
+// zero direction component (1/d infinite) an entry is NaN exactly when the
+// origin lies on that box plane, and the fmaxf of the entry distance then
+// ignores it as ray_aabb does.  The one-lane 4-wide walk uses it for every
+// ray.
+__device__ __forceinline__ bool slab_entries_any(const float4& lo, const float4& hi, f3 o, f3 inv,
+                                                 float tmax, float e[3], float& t_enter)
+{
+    const float bmin[3] = {lo.x, lo.y, lo.z}, bmax[3] = {hi.x, hi.y, hi.z};
+    float t0 = 0.0f, t1 = tmax;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float ta = (bmin[a] - get(o, a)) * get(inv, a), tb = (bmax[a] - get(o, a)) * get(inv, a);
+        const bool sw = ta > tb;
+        const float en = sw ? tb : ta, ex = sw ? ta : tb;
+        e[a] = en;
+        t0 = fmaxf(t0, en);
+        t1 = fminf(t1, ex);
+    }
+    t_enter = t0;
+    return !(t0 > t1);
+}
+
+// Least of two per-axis entries as an entry of the union box.  With a
+// finite 1/d every entry is a number (an overflow gives an infinity) and
+// rounding is monotone, so the union's entry is the least of its parts'.
+// With an infinite 1/d an entry is NaN when the origin lies on that box
+// plane; the slab test's fmaxf then ignores it, which is what -inf does,
+// and the union's entry is NaN (ignored) exactly when one of its parts' is.
+__device__ __forceinline__ float union_entry(float a, float b, bool infax)
+{
+    float m = fminf(a, b);
+    if (infax) m = (a != a || b != b) ? -__builtin_inff() : m;
+    return m;
+}
+
+// Entry distance of a half's box from its two slots' per-axis entries.
+__device__ __forceinline__ float half_entry(const float ea[3], const float eb[3], bool infax)
+{
+    float t = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) t = fmaxf(t, union_entry(ea[a], eb[a], infax));
+    return t;
+}
+
+// Entry keys, for the walks that reduce entries across lanes.  A slot's
+// per-axis entry e enters as the key max(0, e) in the bits of the float
+// (a NaN or -0 gives 0): non-negative floats and +inf order as their bit
+// patterns, so the entry of a union box is the integer minimum of its
+// parts' keys per axis and a box's entry distance the integer maximum over
+// its axes.  This holds for a finite 1/d, where every entry is a number
+// and rounding is monotone (max(0, .) commutes with the minimum).  A ray
+// with an infinite 1/d component can meet NaN entries, for which neither
+// holds; the quad and 16-lane walks hand such rays to the binary walk
+// (mesh_raycast), which is the reference walk itself.
 __device__ __forceinline__ uint32_t entry_key(float e) { return e > 0.0f ? __float_as_uint(e) : 0u; }
 
 // Entry key of the union of two slots from their per-axis keys.
@@ -594,32 +639,30 @@ __device__ __forceinline__ void binary_order(const Cand4 c[4], float tl, float t
 }
 
 // Tests the four slots of a node in registers (slot j: box lo[j].xyz ..
-// hi[j].xyz, ref lo[j].w) and puts them in the binary walk's order (1/d
-// finite).
+// hi[j].xyz, ref lo[j].w) and puts them in the binary walk's order
+// (any 1/d; infax: some component of 1/d is infinite).
 __device__ __forceinline__ void order_children(const float4 lo[4], const float4 hi[4], f3 o, f3 inv,
-                                               float tmax, Cand4 p[4], uint32_t& tested)
+                                               bool infax, float tmax, Cand4 p[4], uint32_t& tested)
 {
     Cand4 c[4];
-    uint32_t k[4][3];
+    float e[4][3];
     tested = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const uint32_t ref = __float_as_uint(lo[j].w);
         // an unused slot holds a zero box: its test runs but never counts
         const bool valid = ref != kNoChild;
-        float e[3], te = 0.0f;
-        const bool h = slab_entries(lo[j], hi[j], o, inv, tmax, e, te) && valid;
+        float te = 0.0f;
+        const bool h = slab_entries_any(lo[j], hi[j], o, inv, tmax, e[j], te) && valid;
         tested += valid ? 1u : 0u;
-        k[j][0] = entry_key(e[0]);
-        k[j][1] = entry_key(e[1]);
-        k[j][2] = entry_key(e[2]);
         c[j].t = te;
         c[j].ref = ref;
         c[j].hit = h ? 1u : 0u;
     }
     // a half whose second slot is unused is its first slot (a leaf)
-    const float tl = c[1].ref == kNoChild ? c[0].t : __uint_as_float(half_key(k[0], k[1]));
-    const float tr = c[3].ref == kNoChild ? c[2].t : __uint_as_float(half_key(k[2], k[3]));
+    const float ul = half_entry(e[0], e[1], infax), ur = half_entry(e[2], e[3], infax);
+    const float tl = c[1].ref == kNoChild ? c[0].t : ul;
+    const float tr = c[3].ref == kNoChild ? c[2].t : ur;
     binary_order(c, tl, tr, p);
 }
 
@@ -650,6 +693,7 @@ struct Quad4Trav {
 // being tested and the nearest hit so far.
 struct Walk4 {
     f3 inv;
+    uint32_t infax;
     uint32_t cur;
     float cur_t;
     int sp;
@@ -661,6 +705,7 @@ struct Walk4 {
 __device__ __forceinline__ bool walk4_begin(const GeomArgs& G, Walk4& W, f3 d)
 {
     W.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    W.infax = inf_axis(W.inv) ? 1u : 0u;
     W.cur = G.root4;
     W.cur_t = 0.0f;
     W.sp = 0;
@@ -706,7 +751,8 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
             }
             Cand4 p[4];
             uint32_t tested;
-            order_children(lo, hi, o, W.inv, W.H.found() ? W.H.best : FLT_MAX, p, tested);
+            order_children(lo, hi, o, W.inv, W.infax != 0u, W.H.found() ? W.H.best : FLT_MAX, p,
+                           tested);
             if (COUNT) cnt.aabb += tested;
             const bool h0 = p[0].hit != 0u, h1 = p[1].hit != 0u, h2 = p[2].hit != 0u,
                        h3 = p[3].hit != 0u;
@@ -762,7 +808,6 @@ __device__ __forceinline__ bool mesh_raycast4_lds(const GeomArgs& G, const Lds4T
     float ovf_t[kStackCap - SK];
     Walk4 W;
     if (!walk4_begin(G, W, d)) return false;
-    if (inf_axis(W.inv)) return mesh_raycast<COUNT>(G, o, d, t_out, tri_out, flags, cnt);
     while (walk4_step<COUNT, SK>(G, L, W, o, d, ovf_ref, ovf_t, cnt)) {
     }
     return W.H.result(t_out, tri_out, flags);
