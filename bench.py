@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-shoebox", action="store_true",
+                    help="skip the shoebox room (trace and order-3 image sources vs one host core)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-stress", action="store_true", help="skip the 2M-triangle town")
@@ -98,31 +100,157 @@ def fp32_error_report(mb, src, lis, n_rays, seed, gpu_bins):
             "per_echogram_max_rel": [float("%.3g" % v) for v in per]}
 
 
-def cpu_baseline(mb, src, lis, n_rays, seed, max_seconds):
-    """The oracle (C port of the reference tracer) on the host cores, same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    import numpy as np
-    soup = mb.soup()
-    pos, idx = O.dedup_tris(soup["tri_verts"])
-    mats = [(list(m.absorption), m.scattering, list(m.transmission)) for m in soup["materials"]]
-    om = O.Mesh(pos, idx, soup["tri_material"], mats, 3, soup["tri_surface"], soup["tri_max_edge"])
-    g = O.mesh_geom(om)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    params = [O.trace_params(n_rays, rng_mode=O.RNG_COUNTER, seed=seed, src_index=i,
-                             bin_dt=0.001, max_time=2.0) for i in range(src.shape[0])]
+CPU_SHARE = 16   # host threads of one GPU's share of the box
+
+
+def _time_oracle_jobs(O, g, src, lis, params, threads, min_seconds):
+    """Repeats the jobs on a `threads`-thread pool for at least min_seconds."""
     reps, br, t0 = 0, 0, time.perf_counter()
     while True:
         _, stats = O.trace_jobs_threaded(g, src, lis, params, threads)
         br += int(stats[:, 2].sum())
         reps += 1
-        if time.perf_counter() - t0 >= max_seconds:
+        if time.perf_counter() - t0 >= min_seconds:
             break
-    dt = time.perf_counter() - t0
-    return dict(value=br / dt, unit="bounce-rays/s", cores=threads, kind="port",
-                sample=f"the full tracer workload ({src.shape[0]} sources x {n_rays} rays x "
-                       f"{lis.shape[0]} listeners) repeated {reps}x in {dt:.1f} s, "
-                       f"one job per source on a {threads}-thread pool, counter RNG mode")
+    return br, reps, time.perf_counter() - t0
+
+
+def cpu_baseline(mb, src, lis, n_rays, seed, max_seconds):
+    """The oracle (C restatement of the reference tracer) on the host cores,
+    same workload.  The headline figure: one job per source on a pool of
+    CPU_SHARE threads (one GPU's share of the host), counter RNG.  Beside
+    it: one thread, and the reference's own sequential xorshift RNG, at
+    one thread and CPU_SHARE threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    g = O.mesh_geom(_oracle_mesh(mb))
+    variants = []
+    head = None
+    for mode, mode_name in ((O.RNG_COUNTER, "counter"), (O.RNG_XORSHIFT, "xorshift")):
+        params = [O.trace_params(n_rays, rng_mode=mode, seed=seed, src_index=i, bin_dt=0.001,
+                                 max_time=2.0) for i in range(src.shape[0])]
+        for threads in (CPU_SHARE, 1):
+            secs = max_seconds if (mode == O.RNG_COUNTER and threads == CPU_SHARE) else 1.0
+            # one thread: a bounded sample (the first 4 sources' jobs)
+            jobs = params if threads > 1 else params[:4]
+            br, reps, dt = _time_oracle_jobs(O, g, src, lis, jobs, threads, secs)
+            v = dict(value=br / dt, unit="bounce-rays/s", cores=threads, kind="port", rng=mode_name,
+                     sample=f"{len(jobs)} source jobs x {n_rays} rays x {lis.shape[0]} listeners, "
+                            f"repeated {reps}x in {dt:.1f} s on a {threads}-thread pool")
+            if head is None:
+                head = dict(v)
+            else:
+                variants.append(v)
+    head["variants"] = variants
+    head["host_cpus"] = os.cpu_count()
+    head["note"] = (f"{CPU_SHARE} threads = one GPU's share of the host's cores; "
+                    "os.cpu_count() reports the machine's")
+    return head
+
+
+def shoebox_leg(ctx, dev, reps=50, cpu_seconds=3.0, want_cpu=True):
+    """The shoebox config: a 12 x 8 x 4 m room as 12 triangles, per-surface
+    absorption, one source and one listener, 4096 rays x 3 bands, 64
+    bounces, floor 1e-6, 1 ms x 2 s, plus order-3 image sources on the BVH
+    (4096-ray fan discovery over four rotations, then every chain solved).
+    The tracer on the GPU as the drop-in is called (host buffers in and
+    out) and device-resident, beside the oracle on one host thread."""
+    import numpy as np
+    import torch
+    import omg_hip as H
+    from omg_hip import scenes
+    mb, src, lis, walls = scenes.shoebox(seed=1)
+    scene = H.Scene.from_builder(ctx, mb)
+    params = H.trace_params(4096, 3, max_bounces=64, seed=0xC1, bin_dt=0.001, max_time=2.0,
+                            floor_rel=1e-6)
+    counted = H.trace_world(scene, params, [src], [lis], counters=True)
+    br = counted.counters["bounce_rays"]
+    ts = []
+    for k in range(reps + 2):
+        t0 = time.perf_counter()
+        H.trace_world(scene, params, [src], [lis])
+        if k >= 2:
+            ts.append(time.perf_counter() - t0)
+    host_s = float(np.median(ts))
+    d_src = torch.from_numpy(np.asarray([src], np.float32)).to(dev)
+    d_lis = torch.from_numpy(np.asarray([lis], np.float32)).to(dev)
+    nb = H.nbins(2.0, 0.001)
+    d_bins = torch.empty((1, 1, nb, 3), dtype=torch.float32, device=dev)
+    d_dirs = torch.empty((1, 1, nb, 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    evs = []
+    for k in range(reps + 2):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        H.trace_world_async(scene, params, d_src.data_ptr(), 1, d_lis.data_ptr(), 1,
+                            d_bins.data_ptr(), d_dirs.data_ptr(), stream=stream.cuda_stream)
+        e1.record(stream)
+        if k >= 2:
+            evs.append((e0, e1))
+    torch.cuda.synchronize()
+    dev_s = float(np.median([a.elapsed_time(b) for a, b in evs])) * 1e-3
+    disc = H.GpuWorldDiscovery(scene, n_rays=4096)
+    solver = H.GpuWorldSolve(scene)
+
+    def ism():
+        seen, chains, orders = set(), [], []
+        for rot in range(4):
+            ch, od = disc.discover(lis, rot)
+            for c, o in zip(ch, od):
+                key = tuple(int(x) for x in c[:o])
+                if key not in seen:
+                    seen.add(key)
+                    chains.append(c)
+                    orders.append(o)
+        ch = np.vstack([np.full((1, 3), 0xFFFF, np.uint16), np.array(chains, np.uint16)])
+        od = np.concatenate([[0], orders]).astype(np.uint8)
+        return ch, od, solver.solve_batch([src], ch, od, lis)
+
+    ism()
+    ti = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        ch, od, rec = ism()
+        ti.append(time.perf_counter() - t0)
+    ism_s = float(np.median(ti))
+    leg = {"workload": "shoebox 12 x 8 x 4 m (12 tris), 1 source x 4096 rays x 3 bands, 64 bounces, "
+                       "floor 1e-6, 1 ms x 2 s; order-3 image sources (4 x 4096-ray fan + solve)",
+           "bounce_rays": br,
+           "gpu_trace_ms": host_s * 1e3, "gpu_trace_bounce_rays_per_s": br / host_s,
+           "gpu_trace_device_resident_ms": dev_s * 1e3,
+           "gpu_trace_device_resident_bounce_rays_per_s": br / dev_s,
+           "gpu_ism_ms": ism_s * 1e3, "ism_chains": int(len(od) - 1),
+           "ism_valid_paths": int(rec["valid"].sum())}
+    if want_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        om = _oracle_mesh(mb)
+        g = O.mesh_geom(om)
+        op = O.trace_params(4096, rng_mode=O.RNG_COUNTER, seed=0xC1, max_bounces=64,
+                            bin_dt=0.001, max_time=2.0, floor_rel=1e-6)
+        n, obr, t0 = 0, 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds or n == 0:
+            o = O.trace(g, src, [lis], op)
+            obr += int(o["stats"][2])
+            n += 1
+        cpu_s = (time.perf_counter() - t0) / n
+        tab, n_surf = O.surface_table(om)
+        t0 = time.perf_counter()
+        ochains = set()
+        for rot in range(4):
+            c, o = O.mesh_chains(om, lis, 4096, rot)
+            ochains |= {tuple(int(x) for x in cc[:oo]) for cc, oo in zip(c, o)}
+        for key in [()] + sorted(ochains):
+            O.mesh_record(om, tab, n_surf, list(key), len(key), src, lis)
+        cpu_ism_s = time.perf_counter() - t0
+        leg["cpu_baseline"] = {"value": obr / n / cpu_s, "unit": "bounce-rays/s", "cores": 1,
+                               "kind": "port", "trace_ms": cpu_s * 1e3, "ism_ms": cpu_ism_s * 1e3,
+                               "sample": f"the same trace {n}x and the image sources once, "
+                                         "on one host thread"}
+        leg["speedup_trace_vs_cpu_1core"] = cpu_s / host_s
+        leg["speedup_ism_vs_cpu_1core"] = cpu_ism_s / ism_s
+    scene.close()
+    return leg
 
 
 def _oracle_mesh(mb):
@@ -337,6 +465,33 @@ def batched_vs_plain(ctx, launches=3, fixed_shape=None):
         H.trace_world(q.scene, q.params, q.sources, q.listeners)
     plain_s = time.perf_counter() - t0
     b.close()
+    # the plain path device-resident: inputs and outputs already in HBM, one
+    # omg_trace_world_async per query back to back on one stream
+    import torch
+    dev = torch.device("cuda", 0)
+    bufs = []
+    for q in Q:
+        ns, nl = len(q.sources), len(q.listeners)
+        nbq = H.nbins(q.params.max_time, q.params.bin_dt)
+        bufs.append((torch.from_numpy(np.ascontiguousarray(q.sources, np.float32)).to(dev),
+                     torch.from_numpy(np.ascontiguousarray(q.listeners, np.float32)).to(dev),
+                     torch.empty((ns, nl, nbq, q.params.nbands), dtype=torch.float32, device=dev),
+                     torch.empty((ns, nl, nbq, 3), dtype=torch.float32, device=dev)))
+    stream = torch.cuda.Stream(device=dev)
+
+    def device_pass():
+        for q, (d_s, d_l, d_b, d_d) in zip(Q, bufs):
+            H.trace_world_async(q.scene, q.params, d_s.data_ptr(), d_s.shape[0], d_l.data_ptr(),
+                                d_l.shape[0], d_b.data_ptr(), d_d.data_ptr(),
+                                stream=stream.cuda_stream)
+
+    device_pass()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    device_pass()
+    torch.cuda.synchronize()
+    plain_dev_s = time.perf_counter() - t0
+    del bufs
     alg = NODE_BYTES * cnt["aabb_tests"] + PRIM_BYTES * cnt["tri_tests"]
     for s in pool:
         s.close()
@@ -352,8 +507,14 @@ def batched_vs_plain(ctx, launches=3, fixed_shape=None):
                     "roofline_frac": alg / batched_s / 1e9 / HBM_PEAK_GBS,
                     "alg_bytes_per_launch": alg, "traffic": BATCH_TRAFFIC_BYTES},
         "plain": {"s": plain_s, "bounce_rays_per_s": br / plain_s,
-                  "queries_per_s": len(Q) / plain_s},
+                  "queries_per_s": len(Q) / plain_s,
+                  "path": "omg_trace_world per query, host buffers in and out"},
+        "plain_device_resident": {"s": plain_dev_s, "bounce_rays_per_s": br / plain_dev_s,
+                                  "queries_per_s": len(Q) / plain_dev_s,
+                                  "path": "omg_trace_world_async per query back to back, "
+                                          "buffers in HBM"},
         "speedup": plain_s / batched_s,
+        "speedup_vs_device_resident_plain": plain_dev_s / batched_s,
         "query0_rel_diff_vs_plain": rel,
     }
 
@@ -761,6 +922,8 @@ def main():
         torch.cuda.synchronize()
         del x, y
     fixed = [int(x) for x in args.batch_shape.split(",")] if args.batch_shape else None
+    if not args.no_shoebox:
+        out["shoebox"] = shoebox_leg(ctx, dev, want_cpu=not args.no_cpu_baseline)
     if not args.no_batch:
         out["batched_vs_plain"] = batched_vs_plain(ctx, fixed_shape=fixed)
     if not args.no_serve:

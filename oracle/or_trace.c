@@ -377,7 +377,11 @@ static void *pool_worker(void *arg)
         if (i >= pl->n)
             break;
         or_trace_job *j = &pl->jobs[i];
-        uint64_t st = 0;
+        /* xorshift mode: one sequential stream per job, seeded from the seed
+         * and the source index (never zero); the counter mode ignores it */
+        uint64_t st = (j->p.seed ? j->p.seed : 1u) ^ ((uint64_t)j->p.src_index * 0x9E3779B97F4A7C15ull);
+        if (!st)
+            st = 1;
         memset(j->stats, 0, sizeof j->stats);
         or_trace(j->g, j->src, j->lis, j->n_lis, &j->p, &st, j->bins, j->dirs, NULL, NULL, NULL,
                  NULL, j->stats);
