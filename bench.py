@@ -30,14 +30,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "omg-audio_amd"))
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
-# HBM bytes per k_trace launch of this workload, measured with rocprofv3 PMC
-# (FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, scales from a 256 MiB copy
-# kernel), profiles/k_trace_city_block_50k.md
-K_TRACE_TRAFFIC_BYTES = 18.39e6
+# HBM bytes per plain launch of the city-block workload (k_trace for the
+# first 4 bounces + k_trace_resume16 for the rays still alive), measured
+# with rocprofv3 PMC: FETCH_SIZE x 2.9995 + WRITE_SIZE x 1.0000, the scales
+# a 256 MiB copy kernel reads in the same runs
+# (profiles/k_trace_resume16_city_block_50k.md)
+PLAIN_TRAFFIC_BYTES = 76.35e6
 # the same for the k_trace_batch_regen launch of the mixed 256-query
-# workload in the shape the autotuner picks (1,12,6,1,4),
-# profiles/k_trace_batch_regen_mixed_256q.md
-BATCH_TRAFFIC_BYTES = 17.38e9
+# workload in the shape (1,12,6,1,4) (profiles/k_trace_batch_regen_mixed_256q.md)
+BATCH_TRAFFIC_BYTES = 28.35e9
+# VALU issue of the same launches (SQ_INSTS_VALU per launch, and the mean
+# active lanes per VALU instruction, SQ_THREAD_CYCLES_VALU /
+# SQ_ACTIVE_INST_VALU / 64), from the same profiles
+PLAIN_VALU_INSTS = 6.9145e8
+PLAIN_VALU_THREAD_UTIL = 0.468
+BATCH_VALU_INSTS = 7.9213e10
+BATCH_VALU_THREAD_UTIL = 0.315
+# Peak VALU issue: 128 FP32 lanes per CU per clock (the published 157.3
+# TFLOP/s FP32 vector peak over 256 CUs at 2.4 GHz), i.e. two wave64 VALU
+# instructions per CU per clock
+VALU_PEAK_INSTS = 256 * 2 * 2.4e9
 NODE_BYTES = 32           # bytes per AABB test (one child box, crates/omg-gpu/src/layout.rs:96-107)
 PRIM_BYTES = 48           # bytes per triangle test (layout.rs:109-122)
 
@@ -113,6 +125,18 @@ def _time_oracle_jobs(O, g, src, lis, params, threads, min_seconds):
         if time.perf_counter() - t0 >= min_seconds:
             break
     return br, reps, time.perf_counter() - t0
+
+
+def _valu_roofline(insts, seconds, bounce_rays, thread_util, kernel):
+    """VALU issue of a launch against the chip's peak: wave64 instructions
+    per second (instruction count from the kernel's SQ profile, time
+    measured here), instructions per bounce-ray at wave and lane level."""
+    achieved = insts / seconds
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_INSTS,
+            "unit": "wave64 VALU instructions/s", "frac": achieved / VALU_PEAK_INSTS,
+            "insts_per_launch": insts, "wave_insts_per_bounce_ray": insts / bounce_rays,
+            "lane_insts_per_bounce_ray": insts * 64 * thread_util / bounce_rays,
+            "thread_utilization": thread_util, "kernel": kernel}
 
 
 def cpu_baseline(mb, src, lis, n_rays, seed, max_seconds):
@@ -505,7 +529,10 @@ def batched_vs_plain(ctx, launches=3, fixed_shape=None):
                     "queries_per_s": len(Q) / batched_s, "launches": launches,
                     "shape": shape, "tune_s": tune_s,
                     "roofline_frac": alg / batched_s / 1e9 / HBM_PEAK_GBS,
-                    "alg_bytes_per_launch": alg, "traffic": BATCH_TRAFFIC_BYTES},
+                    "alg_bytes_per_launch": alg, "traffic": BATCH_TRAFFIC_BYTES,
+                    "valu_roofline": _valu_roofline(BATCH_VALU_INSTS, batched_s, br,
+                                                    BATCH_VALU_THREAD_UTIL,
+                                                    "k_trace_batch_regen (1,12,6,1,4)")},
         "plain": {"s": plain_s, "bounce_rays_per_s": br / plain_s,
                   "queries_per_s": len(Q) / plain_s,
                   "path": "omg_trace_world per query, host buffers in and out"},
@@ -909,8 +936,13 @@ def main():
                    "aabb_tests": tc["aabb_tests"], "tri_tests": tc["tri_tests"]},
         "dome": {"ray_events_per_step": dome_events, "ms": dome_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": K_TRACE_TRAFFIC_BYTES,
-                     "kernel": "k_trace", "alg_bytes_per_launch": alg_bytes_trace},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": PLAIN_TRAFFIC_BYTES,
+                     "kernel": "k_trace (first 4 bounces) + k_trace_resume16 (the rest), "
+                               "one plain launch",
+                     "alg_bytes_per_launch": alg_bytes_trace},
+        "valu_roofline": _valu_roofline(PLAIN_VALU_INSTS, trace_ms * 1e-3, tc["bounce_rays"],
+                                        PLAIN_VALU_THREAD_UTIL,
+                                        "k_trace + k_trace_resume16, one plain launch"),
         "wall_s": wall,
         "timed_vs_counted_rel_err": rel,
     }
