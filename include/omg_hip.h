@@ -112,13 +112,15 @@ int omg_device_name(omg_ctx *ctx, char *buf, uint32_t len);
  * levels staged in LDS per block (in 64 B paired-node units; a 4-wide node
  * takes two), traversal-stack entries per ray kept in LDS (16 or 32;
  * deeper entries spill to scratch), lanes per ray (1, 4 = a DPP quad per
- * ray, 0 = by launch size) and the BVH width walked (2 = the reference's
- * binary tree, 4 = its 4-wide collapse; scenes whose 4-wide stack would
- * exceed 64 entries keep width 2).  Width 2 reproduces the reference's
- * traversal counts; both widths give the same nearest hits, except that
- * among hits at exactly equal distance (flagged) the winner may differ.
- * Defaults: 512, 16, 0, 4.  A call that asks for traversal counters walks
- * width 2 (the counters are the reference traversal's). */
+ * ray, 16 = a DPP row per ray over the 16-wide nodes, 0 = by launch size)
+ * and the BVH width walked (2 = the reference's binary tree, 4 = its
+ * 4-wide regrouping, 16 = its 16-wide regrouping; a scene without usable
+ * wider nodes keeps the next narrower width).  Every width visits the
+ * binary walk's leaves in its order and gives its nearest hits bit for
+ * bit, ties and their flags included; width 2 also reproduces the
+ * reference's traversal counts.  Defaults: 512, 16, 0, 4.  A call that
+ * asks for traversal counters walks width 2 (the counters are the
+ * reference traversal's). */
 int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds, uint32_t lanes,
                           uint32_t width);
 /* Plain launches (omg_trace_world, omg_trace_world_async, omg_trace_box
