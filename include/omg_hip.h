@@ -312,14 +312,18 @@ typedef struct {
  * quad), traversal-stack entries per ray kept in LDS (16 or 32; 8, 12 or
  * 16 with regen), resident waves per SIMD the grid is sized for (1..8),
  * and regen = 1 to refill lanes whose ray has ended with new rays between
- * bounces (lanes = 1 only).  ms: the sampled time the autotuner measured for it (0 when set
- * by hand). */
+ * bounces, or regen = 2 (one lane per ray, width 4, 12 or 16 entries) to
+ * shade a lane's segment and start its next walk - of the same ray or of a
+ * new one - as soon as most of its wave waits, instead of after the wave's
+ * longest walk.  The valid combinations are listed by the error message
+ * of omg_batch_set_shape.  ms: the sampled time the autotuner measured for
+ * it (0 when set by hand). */
 typedef struct {
     uint32_t lanes;
     uint32_t stack_lds;
     uint32_t waves_per_simd;
     uint32_t regen;
-    uint32_t width;             /* BVH width: 2, or 4 with regen = 1 when every scene has it */
+    uint32_t width;             /* BVH width: 2, or 4 when every scene has it (16: 16-lane rows) */
     float ms;
 } omg_batch_shape;
 

@@ -93,6 +93,7 @@ int capture(omg_batch* b)
     BatchShape sh = b->shape;
     if (b->flags & OMG_BATCH_COUNTERS) {
         sh.width = 2;
+        if (sh.regen > 1) sh.regen = 1;
         if (sh.stack_lds == kDeepStack) sh.stack_lds = 16;
         if (sh.lanes == 16) {
             sh.lanes = 4;
@@ -278,8 +279,9 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
         uint32_t lg = rays ? (uint32_t)std::lround(lg_prims / (double)rays) : 0;
         b->class_key = ((uint64_t)(nbm <= 3 ? 3 : nbm <= 6 ? 6 : 8) << 16) |
                        ((uint64_t)log2_ceil(rays) << 8) | lg;
-        // untuned default: the regenerating walk (4-wide where every scene has it)
-        b->shape = all4 ? BatchShape{1, 12, 6, 1, 4} : BatchShape{1, 8, 8, 1, 2};
+        // untuned default: the regenerating walk (4-wide, rays replaced at
+        // walk steps, where every scene has the 4-wide tree)
+        b->shape = all4 ? BatchShape{1, 12, 6, 2, 4} : BatchShape{1, 8, 8, 1, 2};
         auto it = c->tuned.find(b->class_key);
         if (it != c->tuned.end() && (it->second.width == 2 || (it->second.width == 4 && all4) ||
                                      (it->second.width == 16 && all16)) &&
@@ -389,7 +391,8 @@ extern "C" int omg_batch_set_shape(omg_batch* b, const omg_batch_shape* s)
                     "shape: lanes 1|4 with stack_lds 16|32 (width 2) or 16 (width 4); regen 1 "
                     "with lanes 1, stack_lds 8|12|16 and width 2|4, or lanes 4, stack_lds 16 and "
                     "width 2; lanes 4 with stack_lds 40 and width 4; regen 1 with lanes 16, "
-                    "stack_lds 96 and width 16; waves_per_simd 1..8");
+                    "stack_lds 96 and width 16; regen 2 with lanes 1, stack_lds 12|16 and "
+                    "width 4; waves_per_simd 1..8");
     if (sh.width == 4 && !b->all4) return fail(OMG_ERR_INVALID, "a scene of the batch has no 4-wide tree");
     if (sh.width == 16 && !b->all16)
         return fail(OMG_ERR_INVALID, "a scene of the batch has no usable 16-wide tree");
@@ -450,10 +453,11 @@ extern "C" int omg_batch_tune(omg_batch* b, uint32_t stride, omg_batch_shape* ou
     BatchShape best = b->shape;
     float best_ms = time_shape(best);
     // (lanes, stack_lds, regen, width): a lane or a quad per ray, with and
-    // without regeneration, over the binary or the 4-wide tree; each at
-    // 2..8 resident waves per SIMD
-    const uint32_t modes[16][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
+    // without regeneration (between bounces, or at walk steps), over the
+    // binary or the 4-wide tree; each at 2..8 resident waves per SIMD
+    const uint32_t modes[18][4] = {{1u, 8u, 1u, 2u},  {1u, 12u, 1u, 2u}, {1u, 16u, 1u, 2u},
                                    {1u, 8u, 1u, 4u},  {1u, 12u, 1u, 4u}, {1u, 16u, 1u, 4u},
+                                   {1u, 12u, 2u, 4u}, {1u, 16u, 2u, 4u},
                                    {4u, 16u, 1u, 2u}, {4u, 40u, 1u, 4u}, {1u, 16u, 0u, 2u},
                                    {1u, 32u, 0u, 2u}, {4u, 16u, 0u, 2u}, {4u, 32u, 0u, 2u},
                                    {1u, 16u, 0u, 4u}, {4u, 16u, 0u, 4u}, {4u, 40u, 0u, 4u},

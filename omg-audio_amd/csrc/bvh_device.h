@@ -115,10 +115,18 @@ constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 // lane mask in SGPRs, which the compiler failed to reset for every lane
 // between two raycasts of the quad traversal.
 struct BestHit {
-    float best = FLT_MAX, bu = 0.0f, bv = 0.0f;
-    uint32_t btri = kNoHit, tie = 0;
+    float best = FLT_MAX;
+    uint32_t btri = kNoHit;
+    // the result flags of the winner: bit0 its hit lies within 1e-6
+    // barycentric of an edge, bit1 another triangle was hit at exactly its
+    // distance (three registers instead of keeping the winner's u and v)
+    uint32_t fl = 0;
 
     __device__ __forceinline__ bool found() const { return btri != kNoHit; }
+    __device__ __forceinline__ static uint32_t edge_flag(float u, float v)
+    {
+        return (u < 1e-6f || v < 1e-6f || u + v > 1.0f - 1e-6f) ? 1u : 0u;
+    }
     // strict `t < best`; an exact-t tie with another triangle is flagged.
     // Offered only when `take`; written with selects, not branches.
     __device__ __forceinline__ void offer_if(bool take, float t, uint32_t tri, float u, float v)
@@ -127,9 +135,7 @@ struct BestHit {
         const bool same = take & !better & (t == best) & (tri != btri);
         best = better ? t : best;
         btri = better ? tri : btri;
-        bu = better ? u : bu;
-        bv = better ? v : bv;
-        tie = better ? 0u : (same ? 1u : tie);
+        fl = better ? edge_flag(u, v) : (same ? (fl | 2u) : fl);
     }
     // flags: bit0 winning hit within 1e-6 barycentric of an edge, bit1 exact-t tie
     __device__ __forceinline__ bool result(float& t_out, uint32_t& tri_out, uint32_t& flags) const
@@ -137,7 +143,7 @@ struct BestHit {
         if (!found()) return false;
         t_out = best;
         tri_out = btri;
-        flags = ((bu < 1e-6f || bv < 1e-6f || bu + bv > 1.0f - 1e-6f) ? 1u : 0u) | (tie ? 2u : 0u);
+        flags = fl;
         return true;
     }
 };
@@ -398,16 +404,13 @@ __device__ __forceinline__ void fold_leaf(uint32_t cnt_p, uint32_t hit_q, float 
     const uint32_t w = quad_min(is_m ? q : 4u);
     const bool is_w = q == w;
     const uint32_t wtri = quad_or(is_w ? tri_q : 0u);
-    const float wu = __uint_as_float(quad_or(is_w ? __float_as_uint(u_q) : 0u));
-    const float wv = __uint_as_float(quad_or(is_w ? __float_as_uint(v_q) : 0u));
-    const uint32_t tie_m = quad_or((is_m & !is_w & (tri_q != wtri)) ? 1u : 0u);
-    const uint32_t tie_b = quad_or((hit & (t_q == H.best) & (tri_q != H.btri)) ? 1u : 0u);
+    const uint32_t wedge = quad_or(is_w ? BestHit::edge_flag(u_q, v_q) : 0u);
+    const uint32_t tie_m = quad_or((is_m & !is_w & (tri_q != wtri)) ? 2u : 0u);
+    const uint32_t tie_b = quad_or((hit & (t_q == H.best) & (tri_q != H.btri)) ? 2u : 0u);
     const bool better = !H.found() | (tm < H.best);
     H.best = better ? tm : H.best;
     H.btri = better ? wtri : H.btri;
-    H.bu = better ? wu : H.bu;
-    H.bv = better ? wv : H.bv;
-    H.tie = better ? tie_m : (H.tie | tie_b);
+    H.fl = better ? (wedge | tie_m) : (H.fl | tie_b);
 }
 
 template <bool COUNT, int SK>
@@ -536,9 +539,11 @@ __device__ __forceinline__ Cand4 csel(bool c, const Cand4& a, const Cand4& b)
 
 // Slab test of one slot box that also returns the per-axis entries (the
 // lesser of the two plane distances, as the slab test takes them).  For a
-// finite 1/d (the quad and 16-lane walks hand other rays to the binary walk) the plane
-// distances are numbers or infinities, never NaN, so the swap of the
-// reference's test is a min / max and the result is ray_aabb's.
+// finite 1/d the plane distances are numbers or infinities, never NaN, so
+// the swap of the reference's test is a min / max and the result is
+// ray_aabb's.  (The quad and 16-lane walks hand other rays to the binary
+// walk; the one-lane 4-wide walk takes the reference form, below, in a
+// step where some lane at a node has an infinite 1/d component.)
 __device__ __forceinline__ bool slab_entries(const float4& lo, const float4& hi, f3 o, f3 inv,
                                              float tmax, float e[3], float& t_enter)
 {
@@ -639,8 +644,12 @@ __device__ __forceinline__ void binary_order(const Cand4 c[4], float tl, float t
 }
 
 // Tests the four slots of a node in registers (slot j: box lo[j].xyz ..
-// hi[j].xyz, ref lo[j].w) and puts them in the binary walk's order
-// (any 1/d; infax: some component of 1/d is infinite).
+// hi[j].xyz, ref lo[j].w) and puts them in the binary walk's order.  ANY:
+// the reference form of the slab test and NaN-aware union entries, right
+// for any 1/d (infax: some component of 1/d is infinite); otherwise the
+// min / max form, right when every component of 1/d is finite (no plane
+// distance is then NaN, and both forms give the same hits and entries).
+template <bool ANY>
 __device__ __forceinline__ void order_children(const float4 lo[4], const float4 hi[4], f3 o, f3 inv,
                                                bool infax, float tmax, Cand4 p[4], uint32_t& tested)
 {
@@ -653,14 +662,15 @@ __device__ __forceinline__ void order_children(const float4 lo[4], const float4 
         // an unused slot holds a zero box: its test runs but never counts
         const bool valid = ref != kNoChild;
         float te = 0.0f;
-        const bool h = slab_entries_any(lo[j], hi[j], o, inv, tmax, e[j], te) && valid;
+        const bool h = (ANY ? slab_entries_any(lo[j], hi[j], o, inv, tmax, e[j], te)
+                            : slab_entries(lo[j], hi[j], o, inv, tmax, e[j], te)) && valid;
         tested += valid ? 1u : 0u;
         c[j].t = te;
         c[j].ref = ref;
         c[j].hit = h ? 1u : 0u;
     }
     // a half whose second slot is unused is its first slot (a leaf)
-    const float ul = half_entry(e[0], e[1], infax), ur = half_entry(e[2], e[3], infax);
+    const float ul = half_entry(e[0], e[1], ANY && infax), ur = half_entry(e[2], e[3], ANY && infax);
     const float tl = c[1].ref == kNoChild ? c[0].t : ul;
     const float tr = c[3].ref == kNoChild ? c[2].t : ur;
     binary_order(c, tl, tr, p);
@@ -678,6 +688,7 @@ struct Lds4Trav {
     lds_u32* st_ref;           // [SK][bs]
     lds_f32* st_t;             // [SK][bs]
     uint32_t lane, bs;
+    uint32_t bs_log2;          // the block size is a power of two (256)
 };
 
 struct Quad4Trav {
@@ -689,28 +700,32 @@ struct Quad4Trav {
 };
 
 // Resumable state of the 4-wide walk of one lane: the node in hand, its
-// entry distance, the stack depth, the primitives [pk, pend) of the leaf
-// being tested and the nearest hit so far.
+// entry distance, the stack top, the leaf primitives still to test and the
+// nearest hit so far.  sp is the LDS slot of the next push, depth x block
+// size + lane (the stack is laid out [depth][lane]), so that one register
+// holds both the depth and the lane's column.  pk packs the primitives as a
+// leaf reference does - bits 0-27 the next one, bits 28-30 how many are
+// left - and bit 31 says that some component of 1/d is infinite.
 struct Walk4 {
     f3 inv;
-    uint32_t infax;
     uint32_t cur;
     float cur_t;
-    int sp;
-    uint32_t pk, pend;
+    uint32_t sp;
+    uint32_t pk;
     BestHit H;
+
+    __device__ __forceinline__ bool infax() const { return (pk >> 31) != 0u; }
+    __device__ __forceinline__ uint32_t left() const { return (pk >> kLeafCountShift) & 7u; }
 };
 
 // Starts a walk; false when there is nothing to walk (an empty mesh).
-__device__ __forceinline__ bool walk4_begin(const GeomArgs& G, Walk4& W, f3 d)
+__device__ __forceinline__ bool walk4_begin(const GeomArgs& G, const Lds4Trav& L, Walk4& W, f3 d)
 {
     W.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    W.infax = inf_axis(W.inv) ? 1u : 0u;
+    W.pk = inf_axis(W.inv) ? 0x80000000u : 0u;
     W.cur = G.root4;
     W.cur_t = 0.0f;
-    W.sp = 0;
-    W.pk = 0;
-    W.pend = 0;
+    W.sp = L.lane;
     W.H = BestHit();
     return !G.empty;
 }
@@ -723,15 +738,15 @@ __device__ __forceinline__ bool walk4_begin(const GeomArgs& G, Walk4& W, f3 d)
 // largest leaf.  Nodes, primitives and their order are those of the walk
 // that tests a leaf at once.  The stack's top SK entries live in LDS,
 // deeper ones in ovf.
-template <bool COUNT, int SK>
-__device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L, Walk4& W, f3 o,
-                                           f3 d, uint32_t (&ovf_ref)[kStackCap - SK],
-                                           float (&ovf_t)[kStackCap - SK], TravCount& cnt)
+template <bool COUNT, int SK, bool ANY>
+__device__ __forceinline__ bool walk4_step_t(const GeomArgs& G, const Lds4Trav& L, Walk4& W, f3 o,
+                                             f3 d, uint32_t (&ovf_ref)[kStackCap - SK],
+                                             float (&ovf_t)[kStackCap - SK], TravCount& cnt)
 {
-    if (W.pk == W.pend && !(W.H.found() && W.cur_t >= W.H.best)) {
+    if (W.left() == 0u && !(W.H.found() && W.cur_t >= W.H.best)) {
         if (W.cur & kLeafBit) {
-            W.pk = W.cur & kLeafStartMask;
-            W.pend = W.pk + ((W.cur >> kLeafCountShift) & 7u);
+            // a leaf reference is bit 31, the count in bits 28-30, the first primitive
+            W.pk = (W.pk & 0x80000000u) | (W.cur & 0x7FFFFFFFu);
         } else {
             float4 lo[4], hi[4];
             if (W.cur < L.nc) {
@@ -751,21 +766,22 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
             }
             Cand4 p[4];
             uint32_t tested;
-            order_children(lo, hi, o, W.inv, W.infax != 0u, W.H.found() ? W.H.best : FLT_MAX, p,
-                           tested);
+            order_children<ANY>(lo, hi, o, W.inv, ANY && W.infax(), W.H.found() ? W.H.best : FLT_MAX,
+                                p, tested);
             if (COUNT) cnt.aabb += tested;
             const bool h0 = p[0].hit != 0u, h1 = p[1].hit != 0u, h2 = p[2].hit != 0u,
                        h3 = p[3].hit != 0u;
             if (h0 | h1 | h2 | h3) {
                 auto push = [&](uint32_t ref, float t) {
-                    if (W.sp < SK) {
-                        L.st_ref[W.sp * L.bs + L.lane] = ref;
-                        L.st_t[W.sp * L.bs + L.lane] = t;
+                    if (W.sp < SK * L.bs) {
+                        L.st_ref[W.sp] = ref;
+                        L.st_t[W.sp] = t;
                     } else {
-                        ovf_ref[W.sp - SK] = ref;
-                        ovf_t[W.sp - SK] = t;
+                        const uint32_t k = (W.sp >> L.bs_log2) - SK;
+                        ovf_ref[k] = ref;
+                        ovf_t[k] = t;
                     }
-                    W.sp++;
+                    W.sp += L.bs;
                 };
                 // every hit slot after the first is pushed, the last one deepest
                 if (h3 & (h0 | h1 | h2)) push(p[3].ref, p[3].t);
@@ -778,24 +794,39 @@ __device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L,
             }
         }
     }
-    if (W.pk < W.pend) {
-        Prim p = load_prim(G.prims, W.pk);
+    if (W.left() != 0u) {
+        Prim p = load_prim(G.prims, W.pk & kLeafStartMask);
         if (COUNT) cnt.tri++;
         float t, u, v;
         W.H.offer_if(ray_tri(p, o, d, t, u, v), t, p.tri, u, v);
-        W.pk++;
-        if (W.pk < W.pend) return true;
+        // the next primitive, one fewer left (none left: only the infinite-axis bit stays)
+        W.pk = W.left() > 1u ? W.pk + 1u - (1u << kLeafCountShift) : (W.pk & 0x80000000u);
+        if (W.left() != 0u) return true;
     }
-    if (W.sp == 0) return false;
-    W.sp--;
-    if (W.sp < SK) {
-        W.cur = L.st_ref[W.sp * L.bs + L.lane];
-        W.cur_t = L.st_t[W.sp * L.bs + L.lane];
+    if (W.sp < L.bs) return false;
+    W.sp -= L.bs;
+    if (W.sp < SK * L.bs) {
+        W.cur = L.st_ref[W.sp];
+        W.cur_t = L.st_t[W.sp];
     } else {
-        W.cur = ovf_ref[W.sp - SK];
-        W.cur_t = ovf_t[W.sp - SK];
+        const uint32_t k = (W.sp >> L.bs_log2) - SK;
+        W.cur = ovf_ref[k];
+        W.cur_t = ovf_t[k];
     }
     return true;
+}
+
+// The step in the min / max form unless some active lane has an infinite
+// 1/d component (a wave-uniform choice: such rays are rare, and the whole
+// step is duplicated so that neither form holds the other's registers).
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool walk4_step(const GeomArgs& G, const Lds4Trav& L, Walk4& W, f3 o,
+                                           f3 d, uint32_t (&ovf_ref)[kStackCap - SK],
+                                           float (&ovf_t)[kStackCap - SK], TravCount& cnt)
+{
+    if (__ballot(W.infax()) != 0)
+        return walk4_step_t<COUNT, SK, true>(G, L, W, o, d, ovf_ref, ovf_t, cnt);
+    return walk4_step_t<COUNT, SK, false>(G, L, W, o, d, ovf_ref, ovf_t, cnt);
 }
 
 // Nearest hit over the 4-wide nodes, one lane per ray, stack top in LDS.
@@ -807,7 +838,7 @@ __device__ __forceinline__ bool mesh_raycast4_lds(const GeomArgs& G, const Lds4T
     uint32_t ovf_ref[kStackCap - SK];
     float ovf_t[kStackCap - SK];
     Walk4 W;
-    if (!walk4_begin(G, W, d)) return false;
+    if (!walk4_begin(G, L, W, d)) return false;
     while (walk4_step<COUNT, SK>(G, L, W, o, d, ovf_ref, ovf_t, cnt)) {
     }
     return W.H.result(t_out, tri_out, flags);

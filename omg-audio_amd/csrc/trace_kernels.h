@@ -101,7 +101,8 @@ struct BatchShape {
     uint32_t lanes;           // lanes per ray: 1, 4 or 16
     uint32_t stack_lds;       // stack entries per ray in LDS (see valid_batch_shape)
     uint32_t waves_per_simd;  // resident waves per SIMD the grid is sized for
-    uint32_t regen;           // 1: finished rays are replaced between bounces
+    uint32_t regen;           // 1: finished rays are replaced between bounces; 2: a lane's
+                              // bounces follow each other at walk steps (one lane, 4-wide)
     uint32_t width;           // BVH width walked: 2, 4 or 16 (every scene of the batch has it)
 };
 
@@ -110,11 +111,14 @@ struct BatchShape {
 // entries over the 4-wide tree.  With regen: one lane per ray with 8|12|16
 // entries over either tree, or a quad per ray with 16 (binary tree) or
 // kDeepStack (4-wide tree), or a 16-lane row per ray over the 16-wide
-// tree with kStack16 entries.  A kDeepStack shape is valid only for scenes
-// whose 4-wide tree needs at most kDeepStack stack entries.
+// tree with kStack16 entries.  Regen 2 (rays shaded and replaced at walk
+// steps): one lane per ray over the 4-wide tree with 12|16 entries.  A
+// kDeepStack shape is valid only for scenes whose 4-wide tree needs at
+// most kDeepStack stack entries.
 inline bool valid_batch_shape(const BatchShape& s)
 {
-    if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 1) return false;
+    if (s.waves_per_simd < 1 || s.waves_per_simd > 8 || s.regen > 2) return false;
+    if (s.regen == 2) return s.lanes == 1 && s.width == 4 && (s.stack_lds == 12 || s.stack_lds == 16);
     if (s.lanes == 16 || s.width == 16)
         return s.lanes == 16 && s.width == 16 && s.regen == 1 && s.stack_lds == kStack16;
     if (s.width != 2 && s.width != 4) return false;

@@ -211,6 +211,7 @@ __device__ __forceinline__ void init_trav(LdsTrav& L, lds_float4* lds, uint32_t 
 __device__ __forceinline__ void init_trav(Lds4Trav& L, lds_float4* lds, uint32_t nc, int sk)
 {
     init_lane_trav(L, lds, nc, 8u, sk);
+    L.bs_log2 = 31u - (uint32_t)__builtin_clz(blockDim.x);
 }
 __device__ __forceinline__ void init_trav(QuadTrav& L, lds_float4* lds, uint32_t nc, int sk)
 {
@@ -687,6 +688,103 @@ void k_trace_batch_regen(BatchArgs B)
     if (COUNT && B.counters) flush_counters(B.counters, aabb_acc, tri_acc, iters_acc);
 }
 
+// Regenerating launch with the bounce loop cut at walk steps: a lane whose
+// walk has ended does not wait for the wave's longest walk.  Each iteration
+// the wave takes one step of every lane's 4-wide walk; once at least T
+// lanes of the wave have an ended walk or no ray (or no lane walks), those
+// lanes shade their segment (ray_shade, the same code as ray_step's second
+// half) and start their next walk, of the same ray or of a new one taken
+// from the wave's item.  A lane's walk, hit and shading are those of
+// k_trace_batch_regen, so every ray is bit-identical to it; only the
+// interleaving of the lanes' work differs.  One lane per ray.
+template <int NBM, bool COUNT, int SK, int T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+void k_trace_batch_regen_il(BatchArgs B)
+{
+    extern __shared__ float4 lds[];
+    Lds4Trav L;
+    init_trav(L, (lds_float4*)lds, 0u, SK);
+    const uint32_t lane = threadIdx.x & 63u;
+    RayState<NBM> S;
+    Walk4 Wk;
+    GeomArgs G{};  // the walking ray's geometry (node and primitive arrays)
+    uint32_t ovf_ref[kStackCap - SK];
+    float ovf_t[kStackCap - SK];
+    // mode 0: no ray; 1: walking; 2: walk ended, segment not shaded yet
+    uint32_t mode = 0, qi = 0;
+    uint32_t it_q = 0, it_src = 0, it_next = 0, it_end = 0, more = 1;
+    uint64_t aabb_acc = 0, tri_acc = 0, iters_acc = 0;
+    for (;;) {
+        // every lane of the wave is active here
+        const uint64_t ended = __ballot(mode == 2u);
+        const uint64_t idle = __ballot(mode == 0u);
+        const uint64_t walking = __ballot(mode == 1u);
+        const bool refill = more || it_next < it_end;
+        const uint32_t waiting = (uint32_t)__popcll(ended) + (refill ? (uint32_t)__popcll(idle) : 0u);
+        if (walking == 0 || waiting >= (uint32_t)T) {
+            if (mode == 2u) {
+                const JobView& J = B.jobs[qi];
+                float t;
+                uint32_t tri, fl;
+                Hit h;
+                const bool found = Wk.H.result(t, tri, fl);
+                const bool has = mesh_hit(J, S.pos, S.dir, found, t, tri, fl, h);
+                if (ray_shade<NBM, 1>(J, 0u, J.nbands, S, has, h, nullptr, nullptr)) {
+                    mode = walk4_begin(J.g, L, Wk, S.dir) ? 1u : 2u;
+                } else {
+                    ray_end(J, S, 0u);
+                    if (COUNT) iters_acc += S.bounce;
+                    mode = 0u;
+                }
+            }
+            uint64_t need = __ballot(mode == 0u);
+            while (need != 0 && (it_next < it_end || more)) {
+                if (it_next == it_end) {
+                    uint32_t it = 0;
+                    if (lane == 0) it = atomicAdd(B.next_item, 1u);
+                    it = __builtin_amdgcn_readfirstlane(it);
+                    if (it >= B.n_items) {
+                        more = 0;
+                        break;
+                    }
+                    const BatchItem I = B.items[it];
+                    it_q = I.query;
+                    it_src = I.src;
+                    it_next = I.ray_start;
+                    it_end = I.ray_start + I.count;
+                }
+                const uint32_t avail = it_end - it_next;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+                if (mode == 0u && rank < avail) {
+                    const JobView& J = B.jobs[it_q];
+                    ray_begin(J, it_src, it_next + rank, S);
+                    qi = it_q;
+                    mode = walk4_begin(J.g, L, Wk, S.dir) ? 1u : 2u;
+                }
+                const uint32_t k = (uint32_t)__popcll(need);
+                it_next += k < avail ? k : avail;
+                need = __ballot(mode == 0u);
+            }
+            if (__ballot(mode != 0u) == 0) break;
+            if (mode == 1u) {
+                const JobView& J = B.jobs[qi];
+                G.nodes4 = J.g.nodes4;
+                G.prims = J.g.prims;
+            }
+        }
+        if (mode == 1u) {
+            TravCount tc;
+            if (!walk4_step<COUNT, SK>(G, L, Wk, S.pos, S.dir, ovf_ref, ovf_t, tc)) mode = 2u;
+            if (COUNT) {
+                aabb_acc += tc.aabb;
+                tri_acc += tc.tri;
+            }
+        }
+    }
+    if (COUNT && B.counters) flush_counters(B.counters, aabb_acc, tri_acc, iters_acc);
+}
+
 // Multi-lane variant of the regenerating launch: a group of LANES lanes
 // runs each ray (a DPP quad on the 4-wide or binary tree: the 4-wide walk
 // tests a node's four children on the quad's lanes, the binary walk splits
@@ -943,6 +1041,18 @@ static hipError_t launch_batch_regen(const BatchArgs& B, uint32_t width, uint32_
     return hipGetLastError();
 }
 
+// regen 2: the walk-step regenerating kernel, shading once 48 lanes of a
+// wave wait (24, 32 and 40 measured within 1 % of it on the mixed 256-query
+// workload)
+template <int NBM, bool COUNT, int SK>
+static hipError_t launch_batch_regen_il(const BatchArgs& B, uint32_t blocks, hipStream_t s)
+{
+    size_t lds = (size_t)SK * kTraceBlock * 8;
+    hipLaunchKernelGGL((k_trace_batch_regen_il<NBM, COUNT, SK, 48>), dim3(blocks),
+                       dim3(kTraceBlock), lds, s, B);
+    return hipGetLastError();
+}
+
 template <int NBM, bool COUNT, int SK, int W, int LANES>
 static hipError_t launch_batch_regenL(const BatchArgs& B, uint32_t blocks, hipStream_t s)
 {
@@ -956,6 +1066,9 @@ template <int NBM, bool COUNT>
 static hipError_t launch_batch_nb(const BatchArgs& B, const BatchShape& sh, uint32_t blocks,
                                   hipStream_t s)
 {
+    if (sh.regen == 2)
+        return sh.stack_lds == 16 ? launch_batch_regen_il<NBM, COUNT, 16>(B, blocks, s)
+                                  : launch_batch_regen_il<NBM, COUNT, 12>(B, blocks, s);
     if (sh.regen && sh.lanes == 16) return launch_batch_regenL<NBM, COUNT, kStack16, 16, 16>(B, blocks, s);
     if (sh.regen && sh.lanes == 4)
         return sh.width == 4 ? launch_batch_regenL<NBM, COUNT, kDeepStack, 4, 4>(B, blocks, s)
@@ -1021,7 +1134,9 @@ hipError_t launch_trace_routed(const TraceArgs& A, uint32_t nbm, uint32_t width,
     // the batched kernel's default shape: one lane per ray, 12 LDS stack
     // entries, 6 waves per SIMD, rays regenerated between bounces; a 16-lane
     // row per ray over the 16-wide tree when that is the width walked
-    BatchShape sh = width == 16 ? BatchShape{16u, kStack16, 6u, 1u, 16u} : BatchShape{1u, 12u, 6u, 1u, width};
+    BatchShape sh = width == 16 ? BatchShape{16u, kStack16, 6u, 1u, 16u}
+                    : width == 4 ? BatchShape{1u, 12u, 6u, 2u, 4u}
+                                 : BatchShape{1u, 12u, 6u, 1u, width};
     return launch_trace_batch(B, nbm, sh, n_cu, count, s, 0);
 }
 

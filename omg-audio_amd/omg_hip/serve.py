@@ -44,7 +44,7 @@ class Server:
         cfg.slots = slots
         cfg.max_batch = max_batch
         cfg.latency_rays = latency_rays
-        sh = dict(lanes=1, stack_lds=12, waves_per_simd=6, regen=1, width=4)
+        sh = dict(lanes=1, stack_lds=12, waves_per_simd=6, regen=2, width=4)
         sh.update(shape or {})
         cfg.shape = BatchShape(sh["lanes"], sh["stack_lds"], sh["waves_per_simd"], sh["regen"],
                                sh["width"], 0.0)

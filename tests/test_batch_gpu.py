@@ -149,7 +149,8 @@ def test_batch_query_against_oracle(ctx, pool):
                          [(1, 16, 2, 0, 2), (1, 16, 8, 0, 2), (4, 16, 4, 0, 2), (1, 32, 4, 0, 2),
                           (4, 32, 6, 0, 2), (1, 16, 4, 1, 2), (1, 16, 8, 1, 2), (1, 8, 2, 1, 2),
                           (1, 12, 6, 1, 2), (1, 8, 8, 1, 4), (1, 16, 4, 1, 4), (4, 16, 4, 0, 4),
-                          (1, 16, 6, 0, 4), (4, 16, 6, 1, 2), (4, 40, 7, 1, 4), (4, 40, 4, 0, 4)])
+                          (1, 16, 6, 0, 4), (4, 16, 6, 1, 2), (4, 40, 7, 1, 4), (4, 40, 4, 0, 4),
+                          (1, 12, 6, 2, 4), (1, 16, 4, 2, 4), (1, 12, 1, 2, 4)])
 def test_batch_shapes_and_replays(ctx, pool, lanes, stack, wps, regen, width):
     qs = _queries(pool)
     _say(f"case batch shape lanes={lanes} stack_lds={stack} waves/simd={wps} regen={regen} "
@@ -174,7 +175,7 @@ def test_batch_tune(ctx, pool):
     assert shape["lanes"] in (1, 4) and shape["stack_lds"] in (8, 12, 16, 32, 40)
     assert shape["width"] in (2, 4)
     assert 1 <= shape["waves_per_simd"] <= 8 and shape["ms"] > 0
-    assert shape["regen"] in (0, 1)
+    assert shape["regen"] in (0, 1, 2)
     b.launch()
     for i, q in enumerate(qs):
         _same(b.result(i), _plain(q), f"tuned query {i}")
@@ -213,6 +214,14 @@ def test_batch_refusals(ctx, pool):
         b.set_shape(1, 32, 4, regen=1)   # and keeps at most 16 stack entries in LDS
     with pytest.raises(H.OmgError):
         b.set_shape(1, 32, 4, regen=0, width=4)   # the 4-wide item walk keeps 16 entries
+    with pytest.raises(H.OmgError):
+        b.set_shape(4, 40, 4, regen=2, width=4)   # rays replaced at walk steps: one lane per ray,
+    with pytest.raises(H.OmgError):
+        b.set_shape(1, 12, 4, regen=2, width=2)   # over the 4-wide tree,
+    with pytest.raises(H.OmgError):
+        b.set_shape(1, 8, 4, regen=2, width=4)    # with 12 or 16 stack entries in LDS
+    with pytest.raises(H.OmgError):
+        b.set_shape(1, 12, 4, regen=3, width=4)
     b.close()
 
 
