@@ -127,7 +127,8 @@ int omg_set_launch_config(omg_ctx *ctx, uint32_t node_cache, uint32_t stack_lds,
  * excluded) of at least min_rays rays (rays per source x sources) run on
  * the persistent regenerating kernel of the batched launch instead of the
  * single-job kernel: the same rays and per-ray records, echograms equal up
- * to the order of their f32 sums.  Default 4194304; 0 = never.  No
+ * to the order of their f32 sums.  Default 1048576 (the size from which a
+ * plain launch would trace one lane per ray); 0 = never.  No
  * reference counterpart (the reference has one kernel per call). */
 int omg_set_route_rays(omg_ctx *ctx, uint64_t min_rays);
 /* Plain launches that trace a quad per ray over the 4-wide tree (below the

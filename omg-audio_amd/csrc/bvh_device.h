@@ -790,7 +790,10 @@ __device__ __forceinline__ bool walk4_step_t(const GeomArgs& G, const Lds4Trav& 
                 const Cand4 f = csel(h0, p[0], csel(h1, p[1], csel(h2, p[2], p[3])));
                 W.cur = f.ref;
                 W.cur_t = f.t;
-                return true;
+                // a leaf visited next tests its first primitive in this step
+                // (nothing changes before the next step's cull test)
+                if (!(W.cur & kLeafBit) || (W.H.found() && W.cur_t >= W.H.best)) return true;
+                W.pk = (W.pk & 0x80000000u) | (W.cur & 0x7FFFFFFFu);
             }
         }
     }
@@ -803,17 +806,21 @@ __device__ __forceinline__ bool walk4_step_t(const GeomArgs& G, const Lds4Trav& 
         W.pk = W.left() > 1u ? W.pk + 1u - (1u << kLeafCountShift) : (W.pk & 0x80000000u);
         if (W.left() != 0u) return true;
     }
-    if (W.sp < L.bs) return false;
-    W.sp -= L.bs;
-    if (W.sp < SK * L.bs) {
-        W.cur = L.st_ref[W.sp];
-        W.cur_t = L.st_t[W.sp];
-    } else {
-        const uint32_t k = (W.sp >> L.bs_log2) - SK;
-        W.cur = ovf_ref[k];
-        W.cur_t = ovf_t[k];
+    // pop; entries the current best culls are passed over here rather than
+    // in a step each (a culled entry is skipped unopened either way)
+    for (;;) {
+        if (W.sp < L.bs) return false;
+        W.sp -= L.bs;
+        if (W.sp < SK * L.bs) {
+            W.cur = L.st_ref[W.sp];
+            W.cur_t = L.st_t[W.sp];
+        } else {
+            const uint32_t k = (W.sp >> L.bs_log2) - SK;
+            W.cur = ovf_ref[k];
+            W.cur_t = ovf_t[k];
+        }
+        if (!(W.H.found() && W.cur_t >= W.H.best)) return true;
     }
-    return true;
 }
 
 // The step in the min / max form unless some active lane has an infinite
@@ -991,19 +998,10 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
     const uint32_t q = L.q;
     const uint32_t qshift = (__lane_id() & 63u) & ~3u;
     while (true) {
+        bool at_leaf = false;
         if (!(H.found() && cur_t >= H.best)) {
             if (cur & kLeafBit) {
-                uint32_t start = cur & kLeafStartMask;
-                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
-                uint32_t hit_q = 0, tri_q = 0;
-                float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
-                if (q < cnt_p) {
-                    Prim p = load_prim(G.prims, start + q);
-                    tri_q = p.tri;
-                    hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
-                }
-                if (COUNT && q == 0) cnt.tri += cnt_p;
-                fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                at_leaf = true;
             } else {
                 float4 lo, hi;
                 if (cur < L.nc) {
@@ -1048,14 +1046,40 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                     cur = qb<0>(nr) | qb<1>(nr) | qb<2>(nr) | qb<3>(nr);
                     cur_t = __uint_as_float(qb<0>(nt) | qb<1>(nt) | qb<2>(nt) | qb<3>(nt));
                     sp += nhit - 1;
-                    continue;
+                    // a leaf visited next is tested in this iteration
+                    // (nothing changes before the next iteration's cull test)
+                    if (!(cur & kLeafBit) || (H.found() && cur_t >= H.best)) continue;
+                    at_leaf = true;
                 }
             }
         }
-        if (sp == 0) break;
-        sp--;
-        cur = L.st_ref[sp * L.nq + L.quad];
-        cur_t = L.st_t[sp * L.nq + L.quad];
+        if (at_leaf) {
+            uint32_t start = cur & kLeafStartMask;
+            uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+            uint32_t hit_q = 0, tri_q = 0;
+            float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+            if (q < cnt_p) {
+                Prim p = load_prim(G.prims, start + q);
+                tri_q = p.tri;
+                hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+            }
+            if (COUNT && q == 0) cnt.tri += cnt_p;
+            fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+        }
+        // pop, passing over the entries the current best culls (a culled
+        // entry is skipped unopened either way)
+        bool done = false;
+        for (;;) {
+            if (sp == 0) {
+                done = true;
+                break;
+            }
+            sp--;
+            cur = L.st_ref[sp * L.nq + L.quad];
+            cur_t = L.st_t[sp * L.nq + L.quad];
+            if (!(H.found() && cur_t >= H.best)) break;
+        }
+        if (done) break;
     }
     return H.result(t_out, tri_out, flags);
 }
@@ -1126,19 +1150,10 @@ __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Tra
     const uint32_t r = L.r, q = r & 3u;
     const uint32_t rshift = (__lane_id() & 63u) & ~15u;
     while (true) {
+        bool at_leaf = false;
         if (!(H.found() && cur_t >= H.best)) {
             if (cur & kLeafBit) {
-                uint32_t start = cur & kLeafStartMask;
-                uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
-                uint32_t hit_q = 0, tri_q = 0;
-                float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
-                if (q < cnt_p) {
-                    Prim p = load_prim(G.prims, start + q);
-                    tri_q = p.tri;
-                    hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
-                }
-                if (COUNT && r == 0) cnt.tri += cnt_p;
-                fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+                at_leaf = true;
             } else {
                 float4 lo, hi;
                 if (cur < L.nc) {
@@ -1204,14 +1219,40 @@ __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Tra
                     cur = nr;
                     cur_t = __uint_as_float(nt);
                     sp += nhit - 1;
-                    continue;
+                    // a leaf visited next is tested in this iteration
+                    // (nothing changes before the next iteration's cull test)
+                    if (!(cur & kLeafBit) || (H.found() && cur_t >= H.best)) continue;
+                    at_leaf = true;
                 }
             }
         }
-        if (sp == 0) break;
-        sp--;
-        cur = L.st_ref[sp * L.nrow + L.row];
-        cur_t = L.st_t[sp * L.nrow + L.row];
+        if (at_leaf) {
+            uint32_t start = cur & kLeafStartMask;
+            uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+            uint32_t hit_q = 0, tri_q = 0;
+            float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+            if (q < cnt_p) {
+                Prim p = load_prim(G.prims, start + q);
+                tri_q = p.tri;
+                hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+            }
+            if (COUNT && r == 0) cnt.tri += cnt_p;
+            fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+        }
+        // pop, passing over the entries the current best culls (a culled
+        // entry is skipped unopened either way)
+        bool done = false;
+        for (;;) {
+            if (sp == 0) {
+                done = true;
+                break;
+            }
+            sp--;
+            cur = L.st_ref[sp * L.nrow + L.row];
+            cur_t = L.st_t[sp * L.nrow + L.row];
+            if (!(H.found() && cur_t >= H.best)) break;
+        }
+        if (done) break;
     }
     return H.result(t_out, tri_out, flags);
 }

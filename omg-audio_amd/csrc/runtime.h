@@ -85,8 +85,10 @@ struct omg_ctx {
     uint64_t quad_below = 1ull << 20;   // launches with fewer rays trace a quad per ray
     uint32_t n_cu = 0;                  // compute units (persistent launches size their grid to it)
     // plain launches of at least route_rays rays run on the persistent
-    // regenerating kernel (0: never); its job view, items and cursor
-    uint64_t route_rays = 1ull << 22;
+    // regenerating kernel (0: never); its job view, items and cursor.  The
+    // default is quad_below: a launch too large for a quad per ray runs
+    // persistent, never one lane per ray waiting for its block's longest ray
+    uint64_t route_rays = 1ull << 20;
     omg::DevBuf route_job, route_items, route_cursor;
     // quad-per-ray plain launches hand their rays still alive after
     // handoff_bounce bounces to the 16-lane walk (0: never); the saved

@@ -107,7 +107,7 @@ class Context:
         check(lib().omg_device_name(self.h, buf, 128))
         self.device_name = buf.value.decode()
 
-    def set_route_rays(self, min_rays=1 << 22):
+    def set_route_rays(self, min_rays=1 << 20):
         """Plain launches of at least min_rays rays run on the persistent
         regenerating kernel (0 = never; omg_set_route_rays)."""
         check(lib().omg_set_route_rays(self.h, min_rays))
