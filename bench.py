@@ -32,20 +32,28 @@ sys.path.insert(0, os.path.join(ROOT, "omg-audio_amd"))
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per plain launch of the city-block workload (k_trace for the
 # first 4 bounces + k_trace_resume16 for the rays still alive), measured
-# with rocprofv3 PMC: FETCH_SIZE x 2.9995 + WRITE_SIZE x 1.0000, the scales
-# a 256 MiB copy kernel reads in the same runs
+# with rocprofv3 PMC: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scales a
+# 1 GiB copy kernel (--copy-probe) reads in the same runs
 # (profiles/k_trace_resume16_city_block_50k.md)
-PLAIN_TRAFFIC_BYTES = 76.35e6
-# the same for the k_trace_batch_regen launch of the mixed 256-query
-# workload in the shape (1,12,6,1,4) (profiles/k_trace_batch_regen_mixed_256q.md)
-BATCH_TRAFFIC_BYTES = 28.35e9
+PLAIN_TRAFFIC_BYTES = 52.35e6
+# the same for the k_trace_batch_regen_il launch of the mixed 256-query
+# workload in the shape (1,12,6,2,4) (profiles/k_trace_batch_regen_il_mixed_256q.md)
+BATCH_TRAFFIC_BYTES = 18.43e9
+# the town: the routed plain launch (k_trace_batch_regen_il, one-query
+# batch alike) and the single-job kernel (profiles/k_trace_town_2m.md)
+STRESS_TRAFFIC_BYTES = 156.5e9
+STRESS_SINGLE_TRAFFIC_BYTES = 32.6e9
 # VALU issue of the same launches (SQ_INSTS_VALU per launch, and the mean
 # active lanes per VALU instruction, SQ_THREAD_CYCLES_VALU /
 # SQ_ACTIVE_INST_VALU / 64), from the same profiles
-PLAIN_VALU_INSTS = 6.9145e8
-PLAIN_VALU_THREAD_UTIL = 0.468
-BATCH_VALU_INSTS = 7.9213e10
-BATCH_VALU_THREAD_UTIL = 0.315
+PLAIN_VALU_INSTS = 6.593e8
+PLAIN_VALU_THREAD_UTIL = 0.477
+BATCH_VALU_INSTS = 6.610e10
+BATCH_VALU_THREAD_UTIL = 0.377
+STRESS_VALU_INSTS = 5.345e11
+STRESS_VALU_THREAD_UTIL = 0.388
+STRESS_SINGLE_VALU_INSTS = 1.042e12
+STRESS_SINGLE_VALU_THREAD_UTIL = 0.188
 # Peak VALU issue: 128 FP32 lanes per CU per clock (the published 157.3
 # TFLOP/s FP32 vector peak over 256 CUs at 2.4 GHz), i.e. two wave64 VALU
 # instructions per CU per clock
@@ -74,8 +82,8 @@ def parse():
                     help="lanes,stack_lds,waves_per_simd,regen,width for the batched launches "
                          "instead of the autotuner's choice (reproducible profiles)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="after the timed steps, run a plain elementwise copy of 256 MiB "
-                         "(read 256 MiB, write 256 MiB): the calibration kernel of HBM "
+                    help="after the timed steps, run a plain elementwise copy of 1 GiB "
+                         "(read 1 GiB, write 1 GiB): the calibration kernel of HBM "
                          "byte counters")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="minimum wall time of the CPU baseline sample")
@@ -125,6 +133,18 @@ def _time_oracle_jobs(O, g, src, lis, params, threads, min_seconds):
         if time.perf_counter() - t0 >= min_seconds:
             break
     return br, reps, time.perf_counter() - t0
+
+
+def _hbm_use(alg_bytes, traffic_bytes, seconds):
+    """A launch against the HBM roofline: its algorithmic bytes (the
+    reference traversal's node and primitive bytes, most of which the
+    caches serve) and its measured HBM traffic (PMC, from the kernel's
+    profile), each per second as a fraction of the HBM peak."""
+    return {"alg_bytes_per_launch": alg_bytes,
+            "alg_bytes_frac_of_hbm_peak": alg_bytes / seconds / 1e9 / HBM_PEAK_GBS,
+            "traffic_bytes_per_launch": traffic_bytes,
+            "traffic_GBs": traffic_bytes / seconds / 1e9,
+            "traffic_frac_of_hbm_peak": traffic_bytes / seconds / 1e9 / HBM_PEAK_GBS}
 
 
 def _valu_roofline(insts, seconds, bounce_rays, thread_util, kernel):
@@ -528,11 +548,10 @@ def batched_vs_plain(ctx, launches=3, fixed_shape=None):
         "batched": {"s": batched_s, "bounce_rays_per_s": br / batched_s,
                     "queries_per_s": len(Q) / batched_s, "launches": launches,
                     "shape": shape, "tune_s": tune_s,
-                    "roofline_frac": alg / batched_s / 1e9 / HBM_PEAK_GBS,
-                    "alg_bytes_per_launch": alg, "traffic": BATCH_TRAFFIC_BYTES,
+                    "hbm": _hbm_use(alg, BATCH_TRAFFIC_BYTES, batched_s),
                     "valu_roofline": _valu_roofline(BATCH_VALU_INSTS, batched_s, br,
                                                     BATCH_VALU_THREAD_UTIL,
-                                                    "k_trace_batch_regen (1,12,6,1,4)")},
+                                                    "k_trace_batch_regen_il (1,12,6,2,4)")},
         "plain": {"s": plain_s, "bounce_rays_per_s": br / plain_s,
                   "queries_per_s": len(Q) / plain_s,
                   "path": "omg_trace_world per query, host buffers in and out"},
@@ -807,13 +826,19 @@ def stress(ctx, dev, launches=2, fixed_shape=None):
                              speedup_vs_host=build_s / (build["ms"] * 1e-3)),
         "bounce_rays": br, "aabb_tests": tc["aabb_tests"], "tri_tests": tc["tri_tests"],
         "plain": {"ms": pm * 1e3, "bounce_rays_per_s": br / pm,
-                  "roofline_frac": alg / pm / 1e9 / HBM_PEAK_GBS,
-                  "kernel": "persistent regenerating (omg_trace_world routes jobs of >= 4M rays)"},
+                  "hbm": _hbm_use(alg, STRESS_TRAFFIC_BYTES, pm),
+                  "valu_roofline": _valu_roofline(STRESS_VALU_INSTS, pm, br, STRESS_VALU_THREAD_UTIL,
+                                                  "k_trace_batch_regen_il (1,12,6,2,4)"),
+                  "kernel": "persistent regenerating (omg_trace_world routes jobs of >= 1M rays)"},
         "plain_single_job_kernel": {"ms": sm * 1e3, "bounce_rays_per_s": br / sm,
-                                    "roofline_frac": alg / sm / 1e9 / HBM_PEAK_GBS,
+                                    "hbm": _hbm_use(alg, STRESS_SINGLE_TRAFFIC_BYTES, sm),
+                                    "valu_roofline": _valu_roofline(
+                                        STRESS_SINGLE_VALU_INSTS, sm, br,
+                                        STRESS_SINGLE_VALU_THREAD_UTIL,
+                                        "k_trace one lane per ray, 4-wide (routing off)"),
                                     "timed_vs_counted_rel_err": rel1},
         "batched": {"ms": bs * 1e3, "bounce_rays_per_s": br / bs, "shape": shape,
-                    "roofline_frac": alg / bs / 1e9 / HBM_PEAK_GBS},
+                    "hbm": _hbm_use(alg, STRESS_TRAFFIC_BYTES, bs)},
         "timed_vs_counted_rel_err": rel,
     }
 
@@ -947,7 +972,9 @@ def main():
         "timed_vs_counted_rel_err": rel,
     }
     if args.copy_probe:
-        x = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        # 1 GiB each way: four times the 256 MB Infinity Cache, so that the
+        # copy's reads come from HBM whatever the cache holds
+        x = torch.ones(256 << 20, dtype=torch.float32, device=dev)
         y = torch.empty_like(x)
         for _ in range(2):
             torch.mul(x, 1.0, out=y)
