@@ -143,6 +143,13 @@ int omg_set_route_rays(omg_ctx *ctx, uint64_t min_rays);
  * their f32 sums.  Defaults 4, 32, 6; bounce 0 = off.  No reference
  * counterpart. */
 int omg_set_handoff(omg_ctx *ctx, uint32_t bounce, uint32_t node_cache16, uint32_t waves_per_simd);
+/* Lanes per ray of that second kernel: 16 (one 16-wide node slot per
+ * lane, four rays per wave), 8 (two slots per lane, eight rays per wave:
+ * fewer instructions per ray and step, a longer step) or 0 (the default:
+ * 8 for launches of at least 262144 rays, where the kernel is bound by
+ * instruction issue, else 16).  The same rays and per-ray records either
+ * way.  No reference counterpart. */
+int omg_set_handoff_lanes(omg_ctx *ctx, uint32_t lanes);
 
 /* ------------------------------------------------------------ scenes */
 

@@ -106,6 +106,14 @@ extern "C" int omg_set_route_rays(omg_ctx* c, uint64_t min_rays)
     return OMG_OK;
 }
 
+extern "C" int omg_set_handoff_lanes(omg_ctx* c, uint32_t lanes)
+{
+    if (!c) return fail(OMG_ERR_INVALID, "bad argument");
+    if (lanes != 0 && lanes != 8 && lanes != 16) return fail(OMG_ERR_INVALID, "lanes must be 0, 8 or 16");
+    c->handoff_lanes = lanes;
+    return OMG_OK;
+}
+
 extern "C" int omg_set_handoff(omg_ctx* c, uint32_t bounce, uint32_t node_cache16,
                                uint32_t waves_per_simd)
 {
@@ -166,6 +174,13 @@ void omg::plain_shape(const omg_ctx* c, const omg_scene* s, const omg_trace_para
                 A.handoff_bounce = c->handoff_bounce;
                 A.resume_blocks = c->n_cu * c->handoff_wps;
                 A.node_cache16 = std::min(s->n_nodes16, c->handoff_nc);
+                // 8-lane rows (two slots per lane) from 256 k rays: with that
+                // many rays handed off the kernel is bound by instruction
+                // issue, below it by each ray's chain of steps (C2, 64 k
+                // rays: 16 lanes 1.47 ms, 8 lanes 1.51 ms; 256 k rays:
+                // 3.98 / 3.72 ms)
+                A.resume_lanes = c->handoff_lanes ? c->handoff_lanes : rays >= (1u << 18) ? 8u : 16u;
+                A.resume_stack = std::max(1u, s->stack16);
             }
         }
     }

@@ -1257,6 +1257,167 @@ __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Tra
     return H.result(t_out, tri_out, flags);
 }
 
+// The same walk on an 8-lane row: lane r loads and tests slots 2r and 2r+1
+// (64 B), so a wave walks eight rays where the 16-lane row walks four.  The
+// group levels are the 16-lane walk's with the first one inside the lane:
+// slot s = 2r + j; pairs of slots are a lane, groups of 4 two lanes (xor
+// 1), groups of 8 a quad (xor 2), and the two 8-groups the row's halves
+// (row_half_mirror).  Ranks, pushes, culling and the leaf fold are the
+// 16-lane walk's, so the nearest hit is the binary walk's bit for bit.
+struct Row8Trav {
+    const lds_float4* nodes;   // [nc][32]
+    uint32_t nc;
+    lds_u32* st_ref;
+    lds_f32* st_t;
+    uint32_t row, nrow, r;     // row in the block, rows per block, lane in the row
+};
+
+// Bits 0-7 of x moved to the even bit positions 0, 2, ..., 14.
+__device__ __forceinline__ uint32_t spread8(uint32_t x)
+{
+    x = (x | (x << 4)) & 0x0F0Fu;
+    x = (x | (x << 2)) & 0x3333u;
+    return (x | (x << 1)) & 0x5555u;
+}
+
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast16x2(const GeomArgs& G, const Row8Trav& L, f3 o, f3 d,
+                                                 float& t_out, uint32_t& tri_out, uint32_t& flags,
+                                                 TravCount& cnt)
+{
+    if (G.empty) return false;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    if (inf_axis(inv)) return mesh_raycast<COUNT>(G, o, d, t_out, tri_out, flags, cnt);
+    int sp = 0;
+    uint32_t cur = G.root16;
+    float cur_t = 0.0f;
+    BestHit H;
+    const uint32_t r = L.r, q = r & 3u;
+    const uint32_t rshift = (__lane_id() & 63u) & ~7u;
+    while (true) {
+        bool at_leaf = false;
+        if (!(H.found() && cur_t >= H.best)) {
+            if (cur & kLeafBit) {
+                at_leaf = true;
+            } else {
+                float4 lo0, hi0, lo1, hi1;
+                if (cur < L.nc) {
+                    const lds_float4* src = L.nodes + 32 * cur + 4 * r;
+                    lo0 = lds_ld4(src);
+                    hi0 = lds_ld4(src + 1);
+                    lo1 = lds_ld4(src + 2);
+                    hi1 = lds_ld4(src + 3);
+                } else {
+                    const OMG_GLOBAL float4* src = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes16) + cur) + 4 * r;
+                    lo0 = src[0];
+                    hi0 = src[1];
+                    lo1 = src[2];
+                    hi1 = src[3];
+                }
+                const float tmax = H.found() ? H.best : FLT_MAX;
+                const uint32_t ref0 = __float_as_uint(lo0.w), ref1 = __float_as_uint(lo1.w);
+                const bool valid0 = ref0 != kNoChild, valid1 = ref1 != kNoChild;
+                float e0[3], e1[3], te0 = 0.0f, te1 = 0.0f;
+                const bool h0 = slab_entries(lo0, hi0, o, inv, tmax, e0, te0) && valid0;
+                const bool h1 = slab_entries(lo1, hi1, o, inv, tmax, e1, te1) && valid1;
+                if (COUNT && r == 0) {
+                    const uint32_t v0 = (uint32_t)(__ballot(valid0) >> rshift) & 0xFFu;
+                    const uint32_t v1 = (uint32_t)(__ballot(valid1) >> rshift) & 0xFFu;
+                    cnt.aabb += __popc(v0) + __popc(v1);
+                }
+                // hit mask over the node's 16 slots, slot 2r + j at bit 2r + j
+                const uint32_t hm = spread8((uint32_t)(__ballot(h0) >> rshift) & 0xFFu) |
+                                    (spread8((uint32_t)(__ballot(h1) >> rshift) & 0xFFu) << 1);
+                const int nhit = __popc(hm);
+                if (nhit > 0) {
+                    // per-axis keys of the lane's pair, then of its groups of 4 and 8
+                    // (an unused slot has no box: +inf)
+                    const uint32_t inf = 0x7F800000u;
+                    uint32_t a[3], b[3], c[3];
+#pragma unroll
+                    for (int x = 0; x < 3; x++) {
+                        const uint32_t k0 = valid0 ? entry_key(e0[x]) : inf;
+                        const uint32_t k1 = valid1 ? entry_key(e1[x]) : inf;
+                        a[x] = umin3(k0, k1);
+                    }
+#pragma unroll
+                    for (int x = 0; x < 3; x++) b[x] = umin3(a[x], dpp_u<kDppXor1>(a[x]));
+#pragma unroll
+                    for (int x = 0; x < 3; x++) c[x] = umin3(b[x], dpp_u<kDppXor2>(b[x]));
+                    const uint32_t g2 = umax3(a[0], a[1], a[2]), g4 = umax3(b[0], b[1], b[2]),
+                                   g8 = umax3(c[0], c[1], c[2]);
+                    const uint32_t g1a = entry_key(te0), g1b = entry_key(te1);
+                    // each level's sibling group (the pair's other slot in the lane)
+                    const uint32_t s2 = dpp_u<kDppXor1>(g2), s4 = dpp_u<kDppXor2>(g4);
+                    const uint32_t s8 = dpp_u<kDppHalfMirror>(g8);
+                    // levels shared by the lane's two slots (slot bits 1, 2, 3 = lane bits 0, 1, 2)
+                    const uint32_t sl = 2u * r;
+                    int rk = 0;
+                    rk += group_far(r & 1u, g2, s2) ? __popc(hm & (0x3u << (sl ^ 2u))) : 0;
+                    rk += group_far(r & 2u, g4, s4) ? __popc(hm & (0xFu << ((sl & ~3u) ^ 4u))) : 0;
+                    rk += group_far(r & 4u, g8, s8) ? __popc(hm & (0xFFu << ((sl & ~7u) ^ 8u))) : 0;
+                    // slot level: slot 1 of the pair is the right one
+                    const int r0 = rk + (group_far(false, g1a, g1b) ? (int)((hm >> (sl + 1u)) & 1u) : 0);
+                    const int r1 = rk + (group_far(true, g1b, g1a) ? (int)((hm >> sl) & 1u) : 0);
+                    if (h0 && r0 > 0) {
+                        const int pos = sp + nhit - 1 - r0;
+                        L.st_ref[pos * L.nrow + L.row] = ref0;
+                        L.st_t[pos * L.nrow + L.row] = te0;
+                    }
+                    if (h1 && r1 > 0) {
+                        const int pos = sp + nhit - 1 - r1;
+                        L.st_ref[pos * L.nrow + L.row] = ref1;
+                        L.st_t[pos * L.nrow + L.row] = te1;
+                    }
+                    const bool f0 = h0 && r0 == 0, f1 = h1 && r1 == 0;
+                    uint32_t nr = f0 ? ref0 : (f1 ? ref1 : 0u);
+                    uint32_t nt = f0 ? __float_as_uint(te0) : (f1 ? __float_as_uint(te1) : 0u);
+                    nr |= dpp_u<kDppXor1>(nr);
+                    nt |= dpp_u<kDppXor1>(nt);
+                    nr |= dpp_u<kDppXor2>(nr);
+                    nt |= dpp_u<kDppXor2>(nt);
+                    nr |= dpp_u<kDppHalfMirror>(nr);
+                    nt |= dpp_u<kDppHalfMirror>(nt);
+                    cur = nr;
+                    cur_t = __uint_as_float(nt);
+                    sp += nhit - 1;
+                    // a leaf visited next is tested in this iteration
+                    // (nothing changes before the next iteration's cull test)
+                    if (!(cur & kLeafBit) || (H.found() && cur_t >= H.best)) continue;
+                    at_leaf = true;
+                }
+            }
+        }
+        if (at_leaf) {
+            uint32_t start = cur & kLeafStartMask;
+            uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
+            uint32_t hit_q = 0, tri_q = 0;
+            float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+            if (q < cnt_p) {
+                Prim p = load_prim(G.prims, start + q);
+                tri_q = p.tri;
+                hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+            }
+            if (COUNT && r == 0) cnt.tri += cnt_p;
+            fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
+        }
+        // pop, passing over the entries the current best culls
+        bool done = false;
+        for (;;) {
+            if (sp == 0) {
+                done = true;
+                break;
+            }
+            sp--;
+            cur = L.st_ref[sp * L.nrow + L.row];
+            cur_t = L.st_t[sp * L.nrow + L.row];
+            if (!(H.found() && cur_t >= H.best)) break;
+        }
+        if (done) break;
+    }
+    return H.result(t_out, tri_out, flags);
+}
+
 // One crossing of a segment: position along it (0..1), original triangle,
 // authored surface and material.
 struct SegHit {

@@ -96,6 +96,7 @@ struct omg_ctx {
     uint32_t handoff_bounce = 4;
     uint32_t handoff_nc = 32;           // 16-wide nodes in LDS per block of the resume kernel
     uint32_t handoff_wps = 6;           // its resident waves per SIMD
+    uint32_t handoff_lanes = 0;         // its lanes per ray: 16, 8 (two node slots per lane), 0 by size
     omg::DevBuf handoff, handoff_ctl;
     hipStream_t handoff_stream = nullptr;
     bool closing = false;

@@ -62,6 +62,12 @@ struct TraceArgs : JobView {
     uint32_t* handoff_ctl;
     uint32_t resume_blocks;
     uint32_t node_cache16;
+    // lanes per saved ray in k_trace_resume16: 16 (one slot of the 16-wide
+    // node per lane) or 8 (two slots per lane); 0 means 16
+    uint32_t resume_lanes;
+    // LDS stack entries per row: the scene's worst-case 16-wide stack
+    // (<= kStack16); 0 means kStack16
+    uint32_t resume_stack;
 };
 
 // float4s of one saved ray state (position, distance, direction, bounce,

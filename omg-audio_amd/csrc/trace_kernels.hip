@@ -113,6 +113,13 @@ __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Row16T
     return mesh_raycast16<COUNT, SK>(G, L, o, d, t, tri, fl, c);
 }
 template <bool COUNT, int SK>
+__device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Row8Trav& L, f3 o, f3 d,
+                                                 float& t, uint32_t& tri, uint32_t& fl,
+                                                 TravCount& c)
+{
+    return mesh_raycast16x2<COUNT, SK>(G, L, o, d, t, tri, fl, c);
+}
+template <bool COUNT, int SK>
 __device__ __forceinline__ bool mesh_raycast_any(const GeomArgs& G, const Quad4Trav& L, f3 o, f3 d,
                                                  float& t, uint32_t& tri, uint32_t& fl,
                                                  TravCount& c)
@@ -178,6 +185,10 @@ template <>
 struct TravOf<16, 16> {
     using type = Row16Trav;
 };
+template <>
+struct TravOf<8, 16> {
+    using type = Row8Trav;
+};
 
 // float4s per node of width W (64 B paired node, 128 B 4-wide node, 512 B 16-wide node)
 template <int W>
@@ -230,6 +241,16 @@ __device__ __forceinline__ void init_trav(Row16Trav& L, lds_float4* lds, uint32_
     L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nrow);
     L.row = threadIdx.x >> 4;
     L.r = threadIdx.x & 15u;
+}
+__device__ __forceinline__ void init_trav(Row8Trav& L, lds_float4* lds, uint32_t nc, int sk)
+{
+    L.nc = nc;
+    L.nodes = lds;
+    L.nrow = blockDim.x >> 3;
+    L.st_ref = reinterpret_cast<lds_u32*>(lds + 32u * nc);
+    L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nrow);
+    L.row = threadIdx.x >> 3;
+    L.r = threadIdx.x & 7u;
 }
 
 // State of one ray between bounces (tracer.rs:92-102).
@@ -520,27 +541,29 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 }
 
 // Second stage of a handed-off plain launch: the rays that k_trace saved
-// after its hand-off bounce, each continued by a 16-lane row over the
-// 16-wide nodes until it ends (a ray's later bounces are its own: the
+// after its hand-off bounce, each continued by a row of LANES lanes over
+// the 16-wide nodes until it ends (a ray's later bounces are its own: the
 // counter RNG and the saved state make them the same as in one launch).
-// Persistent: the grid is sized to the resident wave slots and a row whose
-// ray has ended takes the next saved ray from the cursor; dead rows of a
-// wave are ranked by a ballot over their first lanes.  The saved rays are
-// the long-lived ones, so a row per ray cuts each bounce's dependent chain
-// of node fetches (four binary levels per fetch) where the early, crowded
-// bounces use a quad per ray.
-template <int NBM, int SK>
+// LANES = 16: one node slot per lane; LANES = 8: two slots per lane, eight
+// rays per wave.  Persistent: the grid is sized to the resident wave slots
+// and a row whose ray has ended takes the next saved ray from the cursor;
+// dead rows of a wave are ranked by a ballot over their first lanes.  The
+// saved rays are the long-lived ones, so a row per ray cuts each bounce's
+// dependent chain of node fetches (four binary levels per fetch) where the
+// early, crowded bounces use a quad per ray.  The LDS stack holds the
+// scene's worst-case 16-wide stack (A.resume_stack entries per row).
+template <int NBM, int SK, int LANES>
 __global__ __launch_bounds__(256) void k_trace_resume16(TraceArgs A)
 {
     extern __shared__ float4 lds[];
-    Row16Trav L;
-    init_trav(L, (lds_float4*)lds, A.node_cache16, SK);
+    typename TravOf<LANES, 16>::type L;
+    init_trav(L, (lds_float4*)lds, A.node_cache16, (int)A.resume_stack);
     for (uint32_t i = threadIdx.x; i < 32u * A.node_cache16; i += blockDim.x)
         lds[i] = reinterpret_cast<const float4*>(A.g.nodes16)[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t r = threadIdx.x & 15u;
-    const uint64_t before = (1ull << (lane & ~15u)) - 1ull;
+    const uint32_t r = threadIdx.x & (LANES - 1u);
+    const uint64_t before = (1ull << (lane & ~(LANES - 1u))) - 1ull;
     const uint32_t n = *reinterpret_cast<const volatile uint32_t*>(A.handoff_ctl);
     RayState<NBM> S;
     uint32_t live = 0, more = 1;
@@ -562,8 +585,8 @@ __global__ __launch_bounds__(256) void k_trace_resume16(TraceArgs A)
         if (__ballot(live != 0u) == 0) break;
         if (live) {
             TravCount tc;
-            bool cont = ray_step<NBM, true, false, SK, 16>(A, L, r, (uint32_t)NBM, S, A.hit_ids,
-                                                          A.hit_t, tc);
+            bool cont = ray_step<NBM, true, false, SK, LANES>(A, L, r, (uint32_t)NBM, S, A.hit_ids,
+                                                             A.hit_t, tc);
             if (!cont) {
                 ray_end(A, S, r);
                 live = 0u;
@@ -955,9 +978,16 @@ static hipError_t launch_trace_nb(const TraceArgs& A, hipStream_t s)
 template <int NBM>
 static hipError_t launch_resume16_n(const TraceArgs& A, hipStream_t s)
 {
-    size_t lds = (size_t)A.node_cache16 * sizeof(Node16) + (size_t)kStack16 * (kTraceBlock / 16) * 8;
-    hipLaunchKernelGGL((k_trace_resume16<NBM, kStack16>), dim3(A.resume_blocks), dim3(kTraceBlock),
-                       lds, s, A);
+    if (A.resume_stack == 0 || A.resume_stack > kStack16) return hipErrorInvalidValue;
+    const uint32_t lanes = A.resume_lanes == 8 ? 8u : 16u;
+    size_t lds = (size_t)A.node_cache16 * sizeof(Node16) +
+                 (size_t)A.resume_stack * (kTraceBlock / lanes) * 8;
+    if (lanes == 8)
+        hipLaunchKernelGGL((k_trace_resume16<NBM, kStack16, 8>), dim3(A.resume_blocks),
+                           dim3(kTraceBlock), lds, s, A);
+    else
+        hipLaunchKernelGGL((k_trace_resume16<NBM, kStack16, 16>), dim3(A.resume_blocks),
+                           dim3(kTraceBlock), lds, s, A);
     return hipGetLastError();
 }
 

@@ -119,6 +119,12 @@ class Context:
         for waves_per_simd resident waves per SIMD (omg_set_handoff)."""
         check(lib().omg_set_handoff(self.h, bounce, node_cache16, waves_per_simd))
 
+    def set_handoff_lanes(self, lanes=0):
+        """Lanes per ray of the hand-off kernel: 16 (one 16-wide node slot
+        per lane), 8 (two slots per lane) or 0 (8 from 262144 rays, else
+        16; omg_set_handoff_lanes)."""
+        check(lib().omg_set_handoff_lanes(self.h, lanes))
+
     def set_launch_config(self, node_cache=512, stack_lds=16, lanes=0, width=4):
         """Tracer launch shape: top BVH nodes staged in LDS (64 B units), LDS
         stack depth, lanes per ray (1, 4, or 0 = chosen by launch size), BVH

@@ -292,6 +292,7 @@ SIGNATURES = [
                              _vp]),
     ("omg_set_route_rays", _i, [_vp, C.c_uint64]),
     ("omg_set_handoff", _i, [_vp, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("omg_set_handoff_lanes", _i, [_vp, C.c_uint32]),
     ("omg_paths_extract_edges", _i, [_vp, _u32, _vp, _u32, _f32, _u32, _vp, _u32,
                                      C.POINTER(_u32)]),
     ("omg_paths_create", _i, [_vp, _u32, C.POINTER(_vp)]),

@@ -139,19 +139,22 @@ def test_launch_shapes_give_identical_rays(ctx, oracle, shape):
 
 
 @pytest.mark.parametrize("scene_name", ["city", "c1"])
-@pytest.mark.parametrize("shape", [(0, 16, 1, 4, 0), (256, 16, 1, 4, 0), (0, 16, 4, 4, 0),
-                                   (256, 16, 4, 4, 0), (256, 32, 4, 4, 0), (256, 16, 4, 4, 1),
-                                   (256, 16, 4, 4, 8), (0, 16, 16, 16, 0), (512, 16, 16, 16, 0)])
+@pytest.mark.parametrize("shape", [(0, 16, 1, 4, 0, 0), (256, 16, 1, 4, 0, 0), (0, 16, 4, 4, 0, 0),
+                                   (256, 16, 4, 4, 0, 0), (256, 32, 4, 4, 0, 0),
+                                   (256, 16, 4, 4, 1, 16), (256, 16, 4, 4, 8, 16),
+                                   (256, 16, 4, 4, 1, 8), (256, 16, 4, 4, 8, 8),
+                                   (0, 16, 16, 16, 0, 0), (512, 16, 16, 16, 0, 0)])
 def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape):
     """The 4-wide and 16-wide nodes are visited in the binary walk's order,
     so every ray - flagged near-edge or exact-tie hits included - has the
     oracle's record, and every bounce the binary walk's hit id and
     distance.  Shapes (LDS node prefix, stack, lanes, width, hand-off
-    bounce): one lane per ray; a quad with the whole stack in LDS (16
-    requested); a quad with 16 LDS entries and the rest in scratch (32
-    requested); a quad handing its rays to 16-lane rows after 1 or 8
-    bounces; a 16-lane row per ray from the start."""
-    nc, sk, lanes, width, handoff = shape
+    bounce, hand-off lanes): one lane per ray; a quad with the whole stack
+    in LDS (16 requested); a quad with 16 LDS entries and the rest in
+    scratch (32 requested); a quad handing its rays to 16-lane rows (one
+    node slot per lane) or to 8-lane rows (two slots per lane) after 1 or
+    8 bounces; a 16-lane row per ray from the start."""
+    nc, sk, lanes, width, handoff, hlanes = shape
     if scene_name == "city":
         mb, bounds, boxes = scenes.city_block(seed=5, n_buildings=8, target_tris=6000)
         src = np.vstack([scenes.indoor_points(51, 2, boxes),
@@ -161,7 +164,7 @@ def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape)
         mb, s1, l1, _ = scenes.shoebox(seed=1)
         src, lis = np.array([s1]), np.array([l1])
     _say(f"case wide walk {scene_name} node_cache={nc} stack_lds={sk} lanes={lanes} width={width} "
-         f"handoff={handoff}")
+         f"handoff={handoff} handoff_lanes={hlanes}")
     kw = dict(n_rays=1000, nbands=3, seed=41, bin_dt=0.001, max_time=2.0)
     scene = H.Scene.from_builder(ctx, mb)
     tree = scene.export()[3]
@@ -169,10 +172,12 @@ def test_width4_walk_gives_the_binary_walks_hits(ctx, oracle, scene_name, shape)
     try:
         ctx.set_launch_config(nc, sk, lanes, width)
         ctx.set_handoff(handoff)
+        ctx.set_handoff_lanes(hlanes)
         gpu, ids, ts = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)
     finally:
         ctx.set_launch_config()
         ctx.set_handoff()
+        ctx.set_handoff_lanes()
     try:
         ctx.set_launch_config(512, 16, 0, 2)
         ref, ids2, ts2 = H.trace_world_hits(scene, H.trace_params(**kw), src, lis)

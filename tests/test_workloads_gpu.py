@@ -47,6 +47,7 @@ def _defaults(ctx):
     """The drop-in's own launch configuration (other tests change it)."""
     ctx.set_launch_config()
     ctx.set_handoff()
+    ctx.set_handoff_lanes()
     ctx.set_route_rays()
 
 

@@ -47,6 +47,7 @@ def _host(out):
 def test_noalloc_matches_async_on_consecutive_calls(ctx):
     ctx.set_launch_config()
     ctx.set_handoff()
+    ctx.set_handoff_lanes()
     ctx.set_route_rays()
     dev = torch.device("cuda", 0)
     mb, src, lis = scenes.c2_workload()
