@@ -60,7 +60,8 @@ constexpr uint32_t kLatencyItemRays = 16;
 // First sizes of a slot's device arena and pinned staging.  Freeing device
 // or pinned memory waits for the whole device, so a slot that grew its
 // buffers a step at a time would stall every other slot's batch each time;
-// starting at these sizes (a C5 batch needs a few MB) makes growth rare.
+// starting at these sizes (a batch of a few requests of up to 32 x 16384 rays needs a
+// few MB) makes growth rare.
 constexpr size_t kMinArenaBytes = 16u << 20;
 constexpr size_t kMinStagingBytes = 4u << 20;
 // Sizes given to every slot at omg_server_create when the budget leaves
