@@ -119,15 +119,20 @@ def test_batch_of_extremes(ctx, oracle):
     _say("case batch of extremes")
     b = H.Batch(ctx, qs, records=True)
     assert b.shape["width"] == 4
-    b.set_shape(1, 8, 8, 1, 2)                   # binary walk: bit-identical to the plain rays
-    b.launch()
-    for i, q in enumerate(qs):
-        ref = H.trace_world(q.scene, q.params, q.sources, q.listeners, records=True, counters=True)
-        got = b.result(i)
-        assert np.array_equal(got.records, ref.records), i
-        eb, ed = echogram_rel_errors(got.bins, got.dirs, ref.bins.astype(np.float64),
-                                     ref.dirs.astype(np.float64))
-        assert eb <= ECHO_TOL and ed <= ECHO_TOL, (i, eb, ed)
+    refs = [H.trace_world(q.scene, q.params, q.sources, q.listeners, records=True, counters=True)
+            for q in qs]
+    # the binary walk regenerating between bounces, and the default 4-wide
+    # walk regenerating at walk steps (band capacity 8, an empty mesh whose
+    # walks end before they start, a one-ray query)
+    for shape in [(1, 8, 8, 1, 2), (1, 12, 6, 2, 4)]:
+        b.set_shape(*shape)
+        b.launch()
+        for i, ref in enumerate(refs):
+            got = b.result(i)
+            assert np.array_equal(got.records, ref.records), (shape, i)
+            eb, ed = echogram_rel_errors(got.bins, got.dirs, ref.bins.astype(np.float64),
+                                         ref.dirs.astype(np.float64))
+            assert eb <= ECHO_TOL and ed <= ECHO_TOL, (shape, i, eb, ed)
     b.close()
     for s in (se, s8s, sc):
         s.close()

@@ -171,6 +171,23 @@ def test_city_block_full_workload(ctx, oracle):
         assert np.array_equal(g["energy"][ei], o["e"]), f"dome eye {ei}: bin energies differ"
         assert np.array_equal(g["dsum"][ei], o["dsum"]), f"dome eye {ei}: bin directions differ"
     _say(f"  dome: {lis.shape[0]} eyes x 512 rays bit-identical ({info['n_tris']} tris)")
+    # four times the rays: the hand-off runs on 8-lane rows (two node slots
+    # per lane) from 262144 rays; every ray equals the 16-lane rows', and
+    # the first 4096 rays of each source are the 4096-ray job's above
+    big = H.trace_params(16384, **kw)
+    rows8 = H.trace_world(scene, big, src, lis, records=True)
+    try:
+        ctx.set_handoff_lanes(16)
+        rows16 = H.trace_world(scene, big, src, lis, records=True)
+    finally:
+        ctx.set_handoff_lanes()
+    assert np.array_equal(rows8.records, rows16.records)
+    d = float(np.max(np.abs(rows8.bins - rows16.bins))) / float(np.max(np.abs(rows16.bins)))
+    assert d <= 1e-5, d
+    head = rows8.records.reshape(src.shape[0], 16384, 4)[:, :4096].reshape(-1, 4)
+    assert np.array_equal(head, gpu.records)
+    _say(f"  {src.shape[0]} x 16384 rays: 8-lane rows == 16-lane rows ray for ray "
+         f"(echogram rel diff {d:.1e}), first 4096 rays per source == the oracle-checked job")
     scene.close()
 
 
