@@ -1041,10 +1041,12 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                         L.st_ref[pos * L.nq + L.quad] = ref;
                         L.st_t[pos * L.nq + L.quad] = te;
                     }
-                    const bool first = h && rank == 0;
-                    const uint32_t nr = first ? ref : 0u, nt = first ? __float_as_uint(te) : 0u;
-                    cur = qb<0>(nr) | qb<1>(nr) | qb<2>(nr) | qb<3>(nr);
-                    cur_t = __uint_as_float(qb<0>(nt) | qb<1>(nt) | qb<2>(nt) | qb<3>(nt));
+                    // the quad's one hit slot of rank 0 is the next node,
+                    // read from its lane by the whole quad (all of it active)
+                    const uint32_t fm = (uint32_t)(__ballot(h && rank == 0) >> qshift) & 0xFu;
+                    const int src = (int)((qshift + (uint32_t)__builtin_ctz(fm)) << 2);
+                    cur = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)ref);
+                    cur_t = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(te)));
                     sp += nhit - 1;
                     // a leaf visited next is tested in this iteration
                     // (nothing changes before the next iteration's cull test)
@@ -1206,18 +1208,13 @@ __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Tra
                         L.st_ref[pos * L.nrow + L.row] = ref;
                         L.st_t[pos * L.nrow + L.row] = te;
                     }
-                    const bool first = h && rank == 0;
-                    uint32_t nr = first ? ref : 0u, nt = first ? __float_as_uint(te) : 0u;
-                    nr |= dpp_u<kDppXor1>(nr);
-                    nt |= dpp_u<kDppXor1>(nt);
-                    nr |= dpp_u<kDppXor2>(nr);
-                    nt |= dpp_u<kDppXor2>(nt);
-                    nr |= dpp_u<kDppHalfMirror>(nr);
-                    nt |= dpp_u<kDppHalfMirror>(nt);
-                    nr |= dpp_u<kDppMirror>(nr);
-                    nt |= dpp_u<kDppMirror>(nt);
-                    cur = nr;
-                    cur_t = __uint_as_float(nt);
+                    // the row's one hit slot of rank 0 is the next node: every
+                    // lane of the row reads it from that lane (the whole row
+                    // is active here, so the source lane is)
+                    const uint32_t fm = (uint32_t)(__ballot(h && rank == 0) >> rshift) & 0xFFFFu;
+                    const int src = (int)((rshift + (uint32_t)__builtin_ctz(fm)) << 2);
+                    cur = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)ref);
+                    cur_t = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(te)));
                     sp += nhit - 1;
                     // a leaf visited next is tested in this iteration
                     // (nothing changes before the next iteration's cull test)
@@ -1369,17 +1366,15 @@ __device__ __forceinline__ bool mesh_raycast16x2(const GeomArgs& G, const Row8Tr
                         L.st_ref[pos * L.nrow + L.row] = ref1;
                         L.st_t[pos * L.nrow + L.row] = te1;
                     }
+                    // the lane holding the row's one hit slot of rank 0 gives
+                    // the next node to the whole row (all of it active here)
                     const bool f0 = h0 && r0 == 0, f1 = h1 && r1 == 0;
-                    uint32_t nr = f0 ? ref0 : (f1 ? ref1 : 0u);
-                    uint32_t nt = f0 ? __float_as_uint(te0) : (f1 ? __float_as_uint(te1) : 0u);
-                    nr |= dpp_u<kDppXor1>(nr);
-                    nt |= dpp_u<kDppXor1>(nt);
-                    nr |= dpp_u<kDppXor2>(nr);
-                    nt |= dpp_u<kDppXor2>(nt);
-                    nr |= dpp_u<kDppHalfMirror>(nr);
-                    nt |= dpp_u<kDppHalfMirror>(nt);
-                    cur = nr;
-                    cur_t = __uint_as_float(nt);
+                    const uint32_t nr = f0 ? ref0 : ref1;
+                    const uint32_t nt = f0 ? __float_as_uint(te0) : __float_as_uint(te1);
+                    const uint32_t fm = (uint32_t)(__ballot(f0 || f1) >> rshift) & 0xFFu;
+                    const int src = (int)((rshift + (uint32_t)__builtin_ctz(fm)) << 2);
+                    cur = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)nr);
+                    cur_t = __uint_as_float((uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)nt));
                     sp += nhit - 1;
                     // a leaf visited next is tested in this iteration
                     // (nothing changes before the next iteration's cull test)
