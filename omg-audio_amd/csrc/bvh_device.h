@@ -359,6 +359,7 @@ struct QuadTrav {
     lds_u32* st_ref;           // [SK][quads per block]
     lds_f32* st_t;
     uint32_t quad, nq, q;      // quad index in the block, quads per block, lane in quad
+    uint32_t nq_log2;          // the block size is a power of two (256)
 };
 
 // Quad reductions by DPP (xor 1, then xor 2 inside the quad).  Every lane
@@ -468,8 +469,8 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
                     float far_t = lnear ? b : a;
                     if (sp < SK) {
                         if (q == 0) {
-                            L.st_ref[sp * L.nq + L.quad] = far_ref;
-                            L.st_t[sp * L.nq + L.quad] = far_t;
+                            L.st_ref[((uint32_t)sp << L.nq_log2) + L.quad] = far_ref;
+                            L.st_t[((uint32_t)sp << L.nq_log2) + L.quad] = far_t;
                         }
                     } else {
                         ovf_ref[sp - SK] = far_ref;
@@ -490,8 +491,8 @@ __device__ __forceinline__ bool mesh_raycast_quad(const GeomArgs& G, const QuadT
         if (sp == 0) break;
         sp--;
         if (sp < SK) {
-            cur = L.st_ref[sp * L.nq + L.quad];
-            cur_t = L.st_t[sp * L.nq + L.quad];
+            cur = L.st_ref[((uint32_t)sp << L.nq_log2) + L.quad];
+            cur_t = L.st_t[((uint32_t)sp << L.nq_log2) + L.quad];
         } else {
             cur = ovf_ref[sp - SK];
             cur_t = ovf_t[sp - SK];
@@ -697,6 +698,7 @@ struct Quad4Trav {
     lds_u32* st_ref;           // [SK][quads per block]
     lds_f32* st_t;
     uint32_t quad, nq, q;
+    uint32_t nq_log2;
 };
 
 // Resumable state of the 4-wide walk of one lane: the node in hand, its
@@ -898,8 +900,8 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
     auto push = [&](uint32_t ref, float t) {
         if (sp < SK) {
             if (q == 0) {
-                L.st_ref[sp * L.nq + L.quad] = ref;
-                L.st_t[sp * L.nq + L.quad] = t;
+                L.st_ref[((uint32_t)sp << L.nq_log2) + L.quad] = ref;
+                L.st_t[((uint32_t)sp << L.nq_log2) + L.quad] = t;
             }
         } else {
             ovf_ref[sp - SK] = ref;
@@ -965,8 +967,8 @@ __device__ __forceinline__ bool mesh_raycast4_quad(const GeomArgs& G, const Quad
         if (sp == 0) break;
         sp--;
         if (sp < SK) {
-            cur = L.st_ref[sp * L.nq + L.quad];
-            cur_t = L.st_t[sp * L.nq + L.quad];
+            cur = L.st_ref[((uint32_t)sp << L.nq_log2) + L.quad];
+            cur_t = L.st_t[((uint32_t)sp << L.nq_log2) + L.quad];
         } else {
             cur = ovf_ref[sp - SK];
             cur_t = ovf_t[sp - SK];
@@ -1038,8 +1040,8 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                                      ((slot_far && ((hm >> (q ^ 1u)) & 1u)) ? 1 : 0);
                     if (h && rank > 0) {
                         const int pos = sp + nhit - 1 - rank;
-                        L.st_ref[pos * L.nq + L.quad] = ref;
-                        L.st_t[pos * L.nq + L.quad] = te;
+                        L.st_ref[((uint32_t)pos << L.nq_log2) + L.quad] = ref;
+                        L.st_t[((uint32_t)pos << L.nq_log2) + L.quad] = te;
                     }
                     // the quad's one hit slot of rank 0 is the next node,
                     // read from its lane by the whole quad (all of it active)
@@ -1077,8 +1079,8 @@ __device__ __forceinline__ bool mesh_raycast4_quad_deep(const GeomArgs& G, const
                 break;
             }
             sp--;
-            cur = L.st_ref[sp * L.nq + L.quad];
-            cur_t = L.st_t[sp * L.nq + L.quad];
+            cur = L.st_ref[((uint32_t)sp << L.nq_log2) + L.quad];
+            cur_t = L.st_t[((uint32_t)sp << L.nq_log2) + L.quad];
             if (!(H.found() && cur_t >= H.best)) break;
         }
         if (done) break;
@@ -1120,6 +1122,7 @@ struct Row16Trav {
     lds_u32* st_ref;
     lds_f32* st_t;
     uint32_t row, nrow, r;     // row in the block, rows per block, lane in the row
+    uint32_t nrow_log2;        // the block size is a power of two (256)
 };
 
 __device__ __forceinline__ uint32_t umin3(uint32_t a, uint32_t b) { return a < b ? a : b; }
@@ -1205,8 +1208,8 @@ __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Tra
                     rank += group_far(r & 8u, g8, s8) ? __popc(hm & (0xFFu << ((r & ~7u) ^ 8u))) : 0;
                     if (h && rank > 0) {
                         const int pos = sp + nhit - 1 - rank;
-                        L.st_ref[pos * L.nrow + L.row] = ref;
-                        L.st_t[pos * L.nrow + L.row] = te;
+                        L.st_ref[((uint32_t)pos << L.nrow_log2) + L.row] = ref;
+                        L.st_t[((uint32_t)pos << L.nrow_log2) + L.row] = te;
                     }
                     // the row's one hit slot of rank 0 is the next node: every
                     // lane of the row reads it from that lane (the whole row
@@ -1245,8 +1248,8 @@ __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Tra
                 break;
             }
             sp--;
-            cur = L.st_ref[sp * L.nrow + L.row];
-            cur_t = L.st_t[sp * L.nrow + L.row];
+            cur = L.st_ref[((uint32_t)sp << L.nrow_log2) + L.row];
+            cur_t = L.st_t[((uint32_t)sp << L.nrow_log2) + L.row];
             if (!(H.found() && cur_t >= H.best)) break;
         }
         if (done) break;
@@ -1267,6 +1270,7 @@ struct Row8Trav {
     lds_u32* st_ref;
     lds_f32* st_t;
     uint32_t row, nrow, r;     // row in the block, rows per block, lane in the row
+    uint32_t nrow_log2;        // the block size is a power of two (256)
 };
 
 // Bits 0-7 of x moved to the even bit positions 0, 2, ..., 14.
@@ -1358,13 +1362,13 @@ __device__ __forceinline__ bool mesh_raycast16x2(const GeomArgs& G, const Row8Tr
                     const int r1 = rk + (group_far(true, g1b, g1a) ? (int)((hm >> sl) & 1u) : 0);
                     if (h0 && r0 > 0) {
                         const int pos = sp + nhit - 1 - r0;
-                        L.st_ref[pos * L.nrow + L.row] = ref0;
-                        L.st_t[pos * L.nrow + L.row] = te0;
+                        L.st_ref[((uint32_t)pos << L.nrow_log2) + L.row] = ref0;
+                        L.st_t[((uint32_t)pos << L.nrow_log2) + L.row] = te0;
                     }
                     if (h1 && r1 > 0) {
                         const int pos = sp + nhit - 1 - r1;
-                        L.st_ref[pos * L.nrow + L.row] = ref1;
-                        L.st_t[pos * L.nrow + L.row] = te1;
+                        L.st_ref[((uint32_t)pos << L.nrow_log2) + L.row] = ref1;
+                        L.st_t[((uint32_t)pos << L.nrow_log2) + L.row] = te1;
                     }
                     // the lane holding the row's one hit slot of rank 0 gives
                     // the next node to the whole row (all of it active here)
@@ -1404,8 +1408,8 @@ __device__ __forceinline__ bool mesh_raycast16x2(const GeomArgs& G, const Row8Tr
                 break;
             }
             sp--;
-            cur = L.st_ref[sp * L.nrow + L.row];
-            cur_t = L.st_t[sp * L.nrow + L.row];
+            cur = L.st_ref[((uint32_t)sp << L.nrow_log2) + L.row];
+            cur_t = L.st_t[((uint32_t)sp << L.nrow_log2) + L.row];
             if (!(H.found() && cur_t >= H.best)) break;
         }
         if (done) break;

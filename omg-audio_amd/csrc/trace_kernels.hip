@@ -210,6 +210,7 @@ __device__ __forceinline__ void init_quad_trav(T& L, lds_float4* lds, uint32_t n
     L.nc = nc;
     L.nodes = lds;
     L.nq = blockDim.x >> 2;
+    L.nq_log2 = 29u - (uint32_t)__builtin_clz(blockDim.x);
     L.st_ref = reinterpret_cast<lds_u32*>(lds + nf4 * nc);
     L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nq);
     L.quad = threadIdx.x >> 2;
@@ -237,6 +238,7 @@ __device__ __forceinline__ void init_trav(Row16Trav& L, lds_float4* lds, uint32_
     L.nc = nc;
     L.nodes = lds;
     L.nrow = blockDim.x >> 4;
+    L.nrow_log2 = 27u - (uint32_t)__builtin_clz(blockDim.x);
     L.st_ref = reinterpret_cast<lds_u32*>(lds + 32u * nc);
     L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nrow);
     L.row = threadIdx.x >> 4;
@@ -247,6 +249,7 @@ __device__ __forceinline__ void init_trav(Row8Trav& L, lds_float4* lds, uint32_t
     L.nc = nc;
     L.nodes = lds;
     L.nrow = blockDim.x >> 3;
+    L.nrow_log2 = 28u - (uint32_t)__builtin_clz(blockDim.x);
     L.st_ref = reinterpret_cast<lds_u32*>(lds + 32u * nc);
     L.st_t = reinterpret_cast<lds_f32*>(L.st_ref + sk * L.nrow);
     L.row = threadIdx.x >> 3;
@@ -905,6 +908,8 @@ __global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
 // ------------------------------------------------------------------ launchers
 
 constexpr uint32_t kTraceBlock = 256;
+// the walks' LDS stacks index depth x (groups per block) by a shift
+static_assert((kTraceBlock & (kTraceBlock - 1u)) == 0u, "trace blocks are a power of two");
 
 template <int NBM, bool MESH, bool COUNT, int SK, int LANES, int W>
 static hipError_t launch_trace_l(const TraceArgs& A, hipStream_t s)
