@@ -68,6 +68,13 @@ struct TraceArgs : JobView {
     // LDS stack entries per row: the scene's worst-case 16-wide stack
     // (<= kStack16); 0 means kStack16
     uint32_t resume_stack;
+    // SIMDs of the device (4 per CU): k_trace_resume16 admits at most
+    // ceil(saved rays / (SIMDs x rays per wave)) working waves per SIMD, so
+    // a few saved rays spread over the chip instead of piling onto the
+    // SIMDs whose waves reach the queue first; 0 admits every wave
+    uint32_t resume_simds;
+    // the launch's tag in the per-SIMD admission table (set by the launcher)
+    unsigned long long resume_epoch;
 };
 
 // float4s of one saved ray state (position, distance, direction, bounce,
