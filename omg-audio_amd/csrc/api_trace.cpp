@@ -178,7 +178,7 @@ void omg::plain_shape(const omg_ctx* c, const omg_scene* s, const omg_trace_para
                 // many rays handed off the kernel is bound by instruction
                 // issue, below it by each ray's chain of steps (a 53k-
                 // triangle city block, 16 sources: 64 k rays 16 lanes
-                // 1.47 ms, 8 lanes 1.51 ms; 256 k rays 3.98 / 3.72 ms)
+                // 1.43 ms, 8 lanes 1.47 ms; 256 k rays 3.86 / 3.60 ms)
                 A.resume_lanes = c->handoff_lanes ? c->handoff_lanes : rays >= (1u << 18) ? 8u : 16u;
                 A.resume_stack = std::max(1u, s->stack16);
                 A.resume_simds = c->n_cu * 4u;
