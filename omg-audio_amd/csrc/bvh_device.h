@@ -1140,121 +1140,155 @@ __device__ __forceinline__ bool group_far(bool right, uint32_t mine, uint32_t si
     return right == (kl <= kr);
 }
 
+// Resumable state of a row's walk (the same in every lane of the row): the
+// node in hand, its entry distance, the LDS stack depth and the nearest hit.
+struct Walk16 {
+    f3 inv;
+    uint32_t cur;
+    float cur_t;
+    int sp;
+    BestHit H;
+};
+
+// Starts a row's walk; false when it has already ended: an empty mesh, or a
+// ray with an infinite 1/d component, which the binary walk (the reference
+// walk itself) traces whole here.
+template <bool COUNT>
+__device__ __forceinline__ bool walk16_begin(const GeomArgs& G, Walk16& W, f3 o, f3 d, TravCount& cnt)
+{
+    W.H = BestHit();
+    W.sp = 0;
+    W.cur = G.root16;
+    W.cur_t = 0.0f;
+    if (G.empty) return false;
+    W.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    if (inf_axis(W.inv)) {
+        float t;
+        uint32_t tri, fl;
+        if (mesh_raycast<COUNT>(G, o, d, t, tri, fl, cnt)) {
+            W.H.best = t;
+            W.H.btri = tri;
+            W.H.fl = fl;
+        }
+        return false;
+    }
+    return true;
+}
+
+// One step of a row's walk: the node in hand tested (a leaf chosen next is
+// tested in the same step), or a leaf, then a pop that passes over the
+// entries the current best culls; false once the walk has ended.
+template <bool COUNT, int SK>
+__device__ __forceinline__ bool walk16_step(const GeomArgs& G, const Row16Trav& L, Walk16& W, f3 o, f3 d,
+                                            TravCount& cnt)
+{
+    const uint32_t r = L.r, q = r & 3u;
+    const uint32_t rshift = (__lane_id() & 63u) & ~15u;
+    bool at_leaf = false;
+    if (!(W.H.found() && W.cur_t >= W.H.best)) {
+        if (W.cur & kLeafBit) {
+            at_leaf = true;
+        } else {
+            float4 lo, hi;
+            if (W.cur < L.nc) {
+                const lds_float4* src = L.nodes + 32 * W.cur + 2 * r;
+                lo = lds_ld4(src);
+                hi = lds_ld4(src + 1);
+            } else {
+                const OMG_GLOBAL float4* src = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes16) + W.cur) + 2 * r;
+                lo = src[0];
+                hi = src[1];
+            }
+            const uint32_t ref = __float_as_uint(lo.w);
+            const bool valid = ref != kNoChild;
+            float e[3], te = 0.0f;
+            const bool h = slab_entries(lo, hi, o, W.inv, W.H.found() ? W.H.best : FLT_MAX, e, te) && valid;
+            if (COUNT && r == 0) {
+                uint32_t v = (uint32_t)(__ballot(valid) >> rshift) & 0xFFFFu;
+                cnt.aabb += __popc(v);
+            }
+            const uint32_t hm = (uint32_t)(__ballot(h) >> rshift) & 0xFFFFu;
+            const int nhit = __popc(hm);
+            if (nhit > 0) {
+                // per-axis keys of this slot's groups of 2, 4 and 8 (an
+                // unused slot has no box: +inf)
+                const uint32_t inf = 0x7F800000u;
+                const uint32_t k0 = valid ? entry_key(e[0]) : inf;
+                const uint32_t k1 = valid ? entry_key(e[1]) : inf;
+                const uint32_t k2 = valid ? entry_key(e[2]) : inf;
+                const uint32_t a0 = umin3(k0, dpp_u<kDppXor1>(k0));
+                const uint32_t a1 = umin3(k1, dpp_u<kDppXor1>(k1));
+                const uint32_t a2 = umin3(k2, dpp_u<kDppXor1>(k2));
+                const uint32_t b0 = umin3(a0, dpp_u<kDppXor2>(a0));
+                const uint32_t b1 = umin3(a1, dpp_u<kDppXor2>(a1));
+                const uint32_t b2 = umin3(a2, dpp_u<kDppXor2>(a2));
+                const uint32_t c0 = umin3(b0, dpp_u<kDppHalfMirror>(b0));
+                const uint32_t c1 = umin3(b1, dpp_u<kDppHalfMirror>(b1));
+                const uint32_t c2 = umin3(b2, dpp_u<kDppHalfMirror>(b2));
+                const uint32_t g2 = umax3(a0, a1, a2), g4 = umax3(b0, b1, b2), g8 = umax3(c0, c1, c2);
+                const uint32_t g1 = entry_key(te);
+                // each level's sibling group
+                const uint32_t s1 = dpp_u<kDppXor1>(g1), s2 = dpp_u<kDppXor2>(g2);
+                const uint32_t s4 = dpp_u<kDppHalfMirror>(g4), s8 = dpp_u<kDppMirror>(g8);
+                int rank = 0;
+                rank += group_far(r & 1u, g1, s1) ? (int)((hm >> (r ^ 1u)) & 1u) : 0;
+                rank += group_far(r & 2u, g2, s2) ? __popc(hm & (0x3u << ((r & ~1u) ^ 2u))) : 0;
+                rank += group_far(r & 4u, g4, s4) ? __popc(hm & (0xFu << ((r & ~3u) ^ 4u))) : 0;
+                rank += group_far(r & 8u, g8, s8) ? __popc(hm & (0xFFu << ((r & ~7u) ^ 8u))) : 0;
+                if (h && rank > 0) {
+                    const int pos = W.sp + nhit - 1 - rank;
+                    L.st_ref[((uint32_t)pos << L.nrow_log2) + L.row] = ref;
+                    L.st_t[((uint32_t)pos << L.nrow_log2) + L.row] = te;
+                }
+                // the row's one hit slot of rank 0 is the next node: every
+                // lane of the row reads it from that lane (the whole row
+                // is active here, so the source lane is)
+                const uint32_t fm = (uint32_t)(__ballot(h && rank == 0) >> rshift) & 0xFFFFu;
+                const int src = (int)((rshift + (uint32_t)__builtin_ctz(fm)) << 2);
+                W.cur = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)ref);
+                W.cur_t = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(te)));
+                W.sp += nhit - 1;
+                // a leaf visited next is tested in this step (nothing
+                // changes before the next step's cull test)
+                if (!(W.cur & kLeafBit) || (W.H.found() && W.cur_t >= W.H.best)) return true;
+                at_leaf = true;
+            }
+        }
+    }
+    if (at_leaf) {
+        uint32_t start = W.cur & kLeafStartMask;
+        uint32_t cnt_p = (W.cur >> kLeafCountShift) & 7u;
+        uint32_t hit_q = 0, tri_q = 0;
+        float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
+        if (q < cnt_p) {
+            Prim p = load_prim(G.prims, start + q);
+            tri_q = p.tri;
+            hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
+        }
+        if (COUNT && r == 0) cnt.tri += cnt_p;
+        fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, W.H);
+    }
+    // pop, passing over the entries the current best culls (a culled
+    // entry is skipped unopened either way)
+    for (;;) {
+        if (W.sp == 0) return false;
+        W.sp--;
+        W.cur = L.st_ref[((uint32_t)W.sp << L.nrow_log2) + L.row];
+        W.cur_t = L.st_t[((uint32_t)W.sp << L.nrow_log2) + L.row];
+        if (!(W.H.found() && W.cur_t >= W.H.best)) return true;
+    }
+}
+
 template <bool COUNT, int SK>
 __device__ __forceinline__ bool mesh_raycast16(const GeomArgs& G, const Row16Trav& L, f3 o, f3 d,
                                                float& t_out, uint32_t& tri_out, uint32_t& flags,
                                                TravCount& cnt)
 {
-    if (G.empty) return false;
-    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    if (inf_axis(inv)) return mesh_raycast<COUNT>(G, o, d, t_out, tri_out, flags, cnt);
-    int sp = 0;
-    uint32_t cur = G.root16;
-    float cur_t = 0.0f;
-    BestHit H;
-    const uint32_t r = L.r, q = r & 3u;
-    const uint32_t rshift = (__lane_id() & 63u) & ~15u;
-    while (true) {
-        bool at_leaf = false;
-        if (!(H.found() && cur_t >= H.best)) {
-            if (cur & kLeafBit) {
-                at_leaf = true;
-            } else {
-                float4 lo, hi;
-                if (cur < L.nc) {
-                    const lds_float4* src = L.nodes + 32 * cur + 2 * r;
-                    lo = lds_ld4(src);
-                    hi = lds_ld4(src + 1);
-                } else {
-                    const OMG_GLOBAL float4* src = reinterpret_cast<const OMG_GLOBAL float4*>(glb(G.nodes16) + cur) + 2 * r;
-                    lo = src[0];
-                    hi = src[1];
-                }
-                const uint32_t ref = __float_as_uint(lo.w);
-                const bool valid = ref != kNoChild;
-                float e[3], te = 0.0f;
-                const bool h = slab_entries(lo, hi, o, inv, H.found() ? H.best : FLT_MAX, e, te) && valid;
-                if (COUNT && r == 0) {
-                    uint32_t v = (uint32_t)(__ballot(valid) >> rshift) & 0xFFFFu;
-                    cnt.aabb += __popc(v);
-                }
-                const uint32_t hm = (uint32_t)(__ballot(h) >> rshift) & 0xFFFFu;
-                const int nhit = __popc(hm);
-                if (nhit > 0) {
-                    // per-axis keys of this slot's groups of 2, 4 and 8 (an
-                    // unused slot has no box: +inf)
-                    const uint32_t inf = 0x7F800000u;
-                    const uint32_t k0 = valid ? entry_key(e[0]) : inf;
-                    const uint32_t k1 = valid ? entry_key(e[1]) : inf;
-                    const uint32_t k2 = valid ? entry_key(e[2]) : inf;
-                    const uint32_t a0 = umin3(k0, dpp_u<kDppXor1>(k0));
-                    const uint32_t a1 = umin3(k1, dpp_u<kDppXor1>(k1));
-                    const uint32_t a2 = umin3(k2, dpp_u<kDppXor1>(k2));
-                    const uint32_t b0 = umin3(a0, dpp_u<kDppXor2>(a0));
-                    const uint32_t b1 = umin3(a1, dpp_u<kDppXor2>(a1));
-                    const uint32_t b2 = umin3(a2, dpp_u<kDppXor2>(a2));
-                    const uint32_t c0 = umin3(b0, dpp_u<kDppHalfMirror>(b0));
-                    const uint32_t c1 = umin3(b1, dpp_u<kDppHalfMirror>(b1));
-                    const uint32_t c2 = umin3(b2, dpp_u<kDppHalfMirror>(b2));
-                    const uint32_t g2 = umax3(a0, a1, a2), g4 = umax3(b0, b1, b2), g8 = umax3(c0, c1, c2);
-                    const uint32_t g1 = entry_key(te);
-                    // each level's sibling group
-                    const uint32_t s1 = dpp_u<kDppXor1>(g1), s2 = dpp_u<kDppXor2>(g2);
-                    const uint32_t s4 = dpp_u<kDppHalfMirror>(g4), s8 = dpp_u<kDppMirror>(g8);
-                    int rank = 0;
-                    rank += group_far(r & 1u, g1, s1) ? (int)((hm >> (r ^ 1u)) & 1u) : 0;
-                    rank += group_far(r & 2u, g2, s2) ? __popc(hm & (0x3u << ((r & ~1u) ^ 2u))) : 0;
-                    rank += group_far(r & 4u, g4, s4) ? __popc(hm & (0xFu << ((r & ~3u) ^ 4u))) : 0;
-                    rank += group_far(r & 8u, g8, s8) ? __popc(hm & (0xFFu << ((r & ~7u) ^ 8u))) : 0;
-                    if (h && rank > 0) {
-                        const int pos = sp + nhit - 1 - rank;
-                        L.st_ref[((uint32_t)pos << L.nrow_log2) + L.row] = ref;
-                        L.st_t[((uint32_t)pos << L.nrow_log2) + L.row] = te;
-                    }
-                    // the row's one hit slot of rank 0 is the next node: every
-                    // lane of the row reads it from that lane (the whole row
-                    // is active here, so the source lane is)
-                    const uint32_t fm = (uint32_t)(__ballot(h && rank == 0) >> rshift) & 0xFFFFu;
-                    const int src = (int)((rshift + (uint32_t)__builtin_ctz(fm)) << 2);
-                    cur = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)ref);
-                    cur_t = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(te)));
-                    sp += nhit - 1;
-                    // a leaf visited next is tested in this iteration
-                    // (nothing changes before the next iteration's cull test)
-                    if (!(cur & kLeafBit) || (H.found() && cur_t >= H.best)) continue;
-                    at_leaf = true;
-                }
-            }
+    Walk16 W;
+    if (walk16_begin<COUNT>(G, W, o, d, cnt))
+        while (walk16_step<COUNT, SK>(G, L, W, o, d, cnt)) {
         }
-        if (at_leaf) {
-            uint32_t start = cur & kLeafStartMask;
-            uint32_t cnt_p = (cur >> kLeafCountShift) & 7u;
-            uint32_t hit_q = 0, tri_q = 0;
-            float t_q = 0.0f, u_q = 0.0f, v_q = 0.0f;
-            if (q < cnt_p) {
-                Prim p = load_prim(G.prims, start + q);
-                tri_q = p.tri;
-                hit_q = ray_tri(p, o, d, t_q, u_q, v_q) ? 1u : 0u;
-            }
-            if (COUNT && r == 0) cnt.tri += cnt_p;
-            fold_leaf(cnt_p, hit_q, t_q, tri_q, u_q, v_q, H);
-        }
-        // pop, passing over the entries the current best culls (a culled
-        // entry is skipped unopened either way)
-        bool done = false;
-        for (;;) {
-            if (sp == 0) {
-                done = true;
-                break;
-            }
-            sp--;
-            cur = L.st_ref[((uint32_t)sp << L.nrow_log2) + L.row];
-            cur_t = L.st_t[((uint32_t)sp << L.nrow_log2) + L.row];
-            if (!(H.found() && cur_t >= H.best)) break;
-        }
-        if (done) break;
-    }
-    return H.result(t_out, tri_out, flags);
+    return W.H.result(t_out, tri_out, flags);
 }
 
 // The same walk on an 8-lane row: lane r loads and tests slots 2r and 2r+1
