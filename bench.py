@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 # with rocprofv3 PMC: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scales a
 # 1 GiB copy kernel (--copy-probe) reads in the same runs
 # (profiles/k_trace_resume16_city_block_50k.md)
-PLAIN_TRAFFIC_BYTES = 51.62e6
+PLAIN_TRAFFIC_BYTES = 53.04e6
 # the same for the k_trace_batch_regen_il launch of the mixed 256-query
 # workload in the shape (1,12,6,2,4) (profiles/k_trace_batch_regen_il_mixed_256q.md)
 BATCH_TRAFFIC_BYTES = 18.43e9
@@ -46,7 +46,7 @@ STRESS_SINGLE_TRAFFIC_BYTES = 32.6e9
 # VALU issue of the same launches (SQ_INSTS_VALU per launch, and the mean
 # active lanes per VALU instruction, SQ_THREAD_CYCLES_VALU /
 # SQ_ACTIVE_INST_VALU / 64), from the same profiles
-PLAIN_VALU_INSTS = 6.524e8
+PLAIN_VALU_INSTS = 6.532e8
 PLAIN_VALU_THREAD_UTIL = 0.463
 BATCH_VALU_INSTS = 6.610e10
 BATCH_VALU_THREAD_UTIL = 0.377
