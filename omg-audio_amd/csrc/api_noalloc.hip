@@ -21,7 +21,7 @@ constexpr int kNaMaxDevices = 64;
 
 // Layout of `scratch`: [hand-off counters | materials | overlays | hand-off queue].
 constexpr size_t kNaCtl = 0;
-constexpr size_t kNaMats = 256;
+constexpr size_t kNaMats = ((size_t)kHandoffCtlWords * sizeof(uint32_t) + 255) / 256 * 256;
 constexpr size_t kNaOvl = kNaMats + (size_t)kNaMaxMats * sizeof(MatParams);
 constexpr size_t kNaQueue = kNaOvl + (size_t)OMG_MAX_OVERLAYS * sizeof(Overlay);
 constexpr size_t kNaBytes = kNaQueue + (size_t)kNaMaxRays * kHandoffF4 * sizeof(float4);
@@ -151,7 +151,7 @@ extern "C" int omg_trace_world_async_noalloc(omg_scene* s, const omg_trace_param
     if (A.handoff_bounce) {
         A.handoff = reinterpret_cast<float4*>(d.ptr + kNaQueue);
         A.handoff_ctl = reinterpret_cast<uint32_t*>(d.ptr + kNaCtl);
-        OMG_HIP_CHECK(launch_fill_zero(reinterpret_cast<float*>(A.handoff_ctl), 2, st));
+        OMG_HIP_CHECK(launch_fill_zero(reinterpret_cast<float*>(A.handoff_ctl), kHandoffCtlWords, st));
     }
     const size_t nb_f = (size_t)n_src * n_lis * nbins * s->nbands;
     const size_t nd_f = (size_t)n_src * n_lis * nbins * 3;

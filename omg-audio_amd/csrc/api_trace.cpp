@@ -199,11 +199,11 @@ int omg::setup_handoff(omg_ctx* c, TraceArgs& A, hipStream_t st)
         OMG_HIP_CHECK(hipStreamSynchronize(c->handoff_stream));
     }
     OMG_HIP_CHECK(c->handoff.ensure(bytes));
-    OMG_HIP_CHECK(c->handoff_ctl.ensure(2 * sizeof(uint32_t)));
+    OMG_HIP_CHECK(c->handoff_ctl.ensure(kHandoffCtlWords * sizeof(uint32_t)));
     c->handoff_stream = st;
     A.handoff = c->handoff.as<float4>();
     A.handoff_ctl = c->handoff_ctl.as<uint32_t>();
-    OMG_HIP_CHECK(launch_fill_zero(reinterpret_cast<float*>(A.handoff_ctl), 2, st));
+    OMG_HIP_CHECK(launch_fill_zero(reinterpret_cast<float*>(A.handoff_ctl), kHandoffCtlWords, st));
     return OMG_OK;
 }
 

@@ -56,7 +56,9 @@ struct TraceArgs : JobView {
     // handoff + kHandoffF4 * k, k from handoff_ctl[0]) instead of going on,
     // and k_trace_resume16 (resume_blocks blocks, node_cache16 16-wide nodes
     // in LDS) continues the saved rays, one 16-lane row each, taking them
-    // from the cursor handoff_ctl[1].  Both words start at zero.
+    // from the cursor handoff_ctl[1]; handoff_ctl[2 + s] counts the working
+    // waves of SIMD s (kResumeAdmitWords of them).  All kHandoffCtlWords
+    // words start at zero.
     uint32_t handoff_bounce;
     float4* handoff;
     uint32_t* handoff_ctl;
@@ -73,9 +75,13 @@ struct TraceArgs : JobView {
     // a few saved rays spread over the chip instead of piling onto the
     // SIMDs whose waves reach the queue first; 0 admits every wave
     uint32_t resume_simds;
-    // the launch's tag in the per-SIMD admission table (set by the launcher)
-    unsigned long long resume_epoch;
 };
+
+// Hand-off control words: the saved-ray count, the cursor, then one
+// working-wave counter per SIMD, indexed by the SIMD's hardware ids (XCC,
+// shader engine, shader array, CU, SIMD: 13 bits).
+constexpr uint32_t kResumeAdmitWords = 1u << 13;
+constexpr uint32_t kHandoffCtlWords = 2u + kResumeAdmitWords;
 
 // float4s of one saved ray state (position, distance, direction, bounce,
 // termination, hash, flags, source, ray, up to kMaxBands energies)

@@ -6,7 +6,6 @@
 // bits of every per-ray quantity equal the CPU oracle's.  Randomness comes
 // from a Philox4x32-10 block per (ray, bounce, source): no state crosses
 // lanes, and a ray's path is the same whatever launch shape runs it.
-#include <atomic>
 #include "bvh_device.h"
 
 #pragma clang fp contract(off)
@@ -562,13 +561,12 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs A)
 // waves pile onto a few SIMDs (up to 6 on one, each waiting on the others'
 // issue) takes half again as long as one whose waves spread out.  So a wave
 // works only while its SIMD has fewer than ceil(saved rays / (SIMDs x rays
-// per wave)) working waves, counted in g_resume_admit under the launch's
-// epoch (no clearing between launches); a wave turned away exits.  The
-// first wave on every SIMD is admitted and admitted waves take saved rays
-// until none are left, so every saved ray is continued whatever the
+// per wave)) working waves, counted in the launch's own control words
+// (cleared with the rest of them before every launch, graph replays
+// included); a wave turned away exits.  The first wave on every SIMD is
+// admitted (when there is a saved ray at all) and admitted waves take saved
+// rays until none are left, so every saved ray is continued whatever the
 // placement.
-__device__ unsigned long long g_resume_admit[1u << 13];
-
 __device__ __forceinline__ bool resume_admit(const TraceArgs& A, uint32_t n, uint32_t rays_per_wave)
 {
     if (A.resume_simds == 0) return true;
@@ -582,25 +580,7 @@ __device__ __forceinline__ bool resume_admit(const TraceArgs& A, uint32_t n, uin
         const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);    // XCC_ID
         const uint32_t key = ((xcc & 7u) << 10) | (((hw >> 13) & 7u) << 7) | (((hw >> 12) & 1u) << 6) |
                              (((hw >> 8) & 15u) << 2) | ((hw >> 4) & 3u);
-        // an entry is the epoch of the launch that last counted here (bits
-        // 8-63) and that launch's working waves on the SIMD (bits 0-7)
-        unsigned long long* w = &g_resume_admit[key];
-        const unsigned long long tag = A.resume_epoch << 8;
-        unsigned long long old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (;;) {
-            unsigned long long want;
-            if ((old & ~0xFFull) != tag) {
-                want = tag | 1ull;
-            } else if ((uint32_t)(old & 0xFFull) < cap) {
-                want = old + 1ull;
-            } else {
-                ok = 0;
-                break;
-            }
-            const unsigned long long prev = atomicCAS(w, old, want);
-            if (prev == old) break;
-            old = prev;
-        }
+        ok = atomicAdd(A.handoff_ctl + 2u + key, 1u) < cap ? 1u : 0u;
     }
     return __builtin_amdgcn_readfirstlane(ok) != 0u;
 }
@@ -1031,16 +1011,9 @@ static hipError_t launch_trace_nb(const TraceArgs& A, hipStream_t s)
     }
 }
 
-// Tags of the hand-off launches in the admission table: one count for the
-// process, so no two launches share a tag (the table is per device and
-// shared by every context and stream; 56 bits of tag do not wrap).
-static std::atomic<unsigned long long> g_resume_epoch{0};
-
 template <int NBM>
-static hipError_t launch_resume16_n(const TraceArgs& A0, hipStream_t s)
+static hipError_t launch_resume16_n(const TraceArgs& A, hipStream_t s)
 {
-    TraceArgs A = A0;
-    A.resume_epoch = (g_resume_epoch.fetch_add(1u, std::memory_order_relaxed) + 1u) & ((1ull << 56) - 1u);
     if (A.resume_stack == 0 || A.resume_stack > kStack16) return hipErrorInvalidValue;
     const uint32_t lanes = A.resume_lanes == 8 ? 8u : 16u;
     size_t lds = (size_t)A.node_cache16 * sizeof(Node16) +

@@ -1,10 +1,10 @@
 """The hand-off kernel's per-SIMD admission under concurrent launches.
 
 k_trace_resume16 admits at most ceil(saved rays / (SIMDs x rays per
-wave)) working waves per SIMD, counted in one per-device table under each
-launch's tag, and turns the other waves away.  Two contexts here launch
-plain jobs on two streams back to back without waiting, so their hand-off
-kernels run at the same time and count in the same table: a 4096-ray
+wave)) working waves per SIMD, counted in the launch's own control words,
+and turns the other waves away.  Two contexts here launch plain jobs on
+two streams back to back without waiting, so their hand-off kernels run at
+the same time, each counting in its own context's words: a 4096-ray
 shoebox (every ray alive at the hand-off, one working wave per SIMD), a
 city block of 16 x 512 rays (a few waves per SIMD) and the city block of
 16 x 4096 rays (every wave admitted).  Every launch must continue every

@@ -4,7 +4,8 @@ shares, on the same device buffers: records bit-identical, echograms within
 1e-4 relative (the two sum the same f32 contributions in different atomic
 orders).  Two calls in a row - the city block workload (16 sources x 4096
 rays x 8 listeners), then a smaller job with an overlay panel, whose
-tables differ - then the first job again; the limits are refused.
+tables differ - then the first job again; captured in a HIP graph and
+replayed, it equals its eager call on every replay; the limits are refused.
 """
 import numpy as np
 import pytest
@@ -84,6 +85,59 @@ def test_noalloc_matches_async_on_consecutive_calls(ctx):
     # the third call repeats the first: same rays
     assert np.array_equal(got[0][0], got[2][0])
     scene.close()
+
+
+def test_noalloc_graph_replays_equal_the_eager_call(ctx):
+    """The allocation-free launch captured in a HIP graph (the callers it is
+    for) and replayed three times, with eager work between the replays:
+    every replay equals the eager call ray for ray.  The jobs hand few rays
+    to the 16-lane kernel, so its per-SIMD admission turns waves away and
+    every replay depends on the counters being cleared again."""
+    ctx.set_launch_config()
+    ctx.set_handoff()
+    ctx.set_handoff_lanes()
+    ctx.set_route_rays()
+    dev = torch.device("cuda", 0)
+    mb, s1, l1, _ = scenes.shoebox(seed=1)
+    box = H.Scene.from_builder(ctx, mb)
+    mb, src, lis = scenes.c2_workload()
+    city = H.Scene.from_builder(ctx, mb)
+    jobs = [("shoebox 1 x 4096 rays", box,
+             H.trace_params(4096, 3, max_bounces=64, seed=0xC1, bin_dt=0.001, max_time=2.0,
+                            floor_rel=1e-6), np.asarray([s1], np.float32), np.asarray([l1], np.float32)),
+            ("city block 16 x 512 rays x 8 listeners", city,
+             H.trace_params(512, 3, seed=7, bin_dt=0.001, max_time=2.0), src, lis)]
+    stream = torch.cuda.Stream(device=dev)
+    for label, scene, p, s, l in jobs:
+        d_src = torch.from_numpy(np.ascontiguousarray(s, np.float32)).to(dev)
+        d_lis = torch.from_numpy(np.ascontiguousarray(l, np.float32)).to(dev)
+        e_rec, e_bins, e_dirs = _host(_launch(H.trace_world_async_noalloc, scene, p, d_src, d_lis, (),
+                                              stream))
+        bins, dirs, recs = _buffers(p, d_src.shape[0], d_lis.shape[0])
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            H.trace_world_async_noalloc(scene, p, d_src.data_ptr(), d_src.shape[0], d_lis.data_ptr(),
+                                        d_lis.shape[0], bins.data_ptr(), dirs.data_ptr(),
+                                        stream=stream.cuda_stream, d_records=recs.data_ptr())
+        for rep in range(3):
+            # eager work between the replays: the outputs poisoned again
+            bins.fill_(float("nan"))
+            dirs.fill_(float("nan"))
+            recs.zero_()
+            torch.cuda.synchronize()
+            graph.replay()
+            g_rec, g_bins, g_dirs = _host((bins, dirs, recs))
+            print(f"case no-alloc graph replay {rep}: {label}", flush=True)
+            diff = np.nonzero(np.any(g_rec != e_rec, axis=1))[0]
+            assert diff.size == 0, f"{label}, replay {rep}: {diff.size} rays differ, first {diff[:5]}"
+            assert not np.isnan(g_bins).any() and not np.isnan(g_dirs).any(), (label, rep)
+            eb, ed = echogram_rel_errors(g_bins, g_dirs, e_bins.astype(np.float64),
+                                         e_dirs.astype(np.float64))
+            assert eb <= ECHO_TOL and ed <= ECHO_TOL, (label, rep, eb, ed)
+        del graph
+    box.close()
+    city.close()
 
 
 def test_noalloc_refuses_beyond_its_limits(ctx):
