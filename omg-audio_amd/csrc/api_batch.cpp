@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "runtime.h"
 
@@ -258,11 +259,22 @@ extern "C" int omg_batch_create(omg_ctx* c, const omg_query* queries, uint32_t n
     }
     if (!rc) {
         // queries of one scene are consecutive in the work list, so at any
-        // moment the device walks few BVHs and they stay in L2 / Infinity Cache
+        // moment the device walks few BVHs and they stay in L2 / Infinity
+        // Cache.  Larger scenes come first, and in a scene the queries whose
+        // rays may live longest: the launch ends with the rays taken last,
+        // so the longest work is taken early
         std::vector<uint32_t> order(n);
         for (uint32_t i = 0; i < n; i++) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-            return b->q[x].scene < b->q[y].scene;
+            const omg_query& qx = queries[x];
+            const omg_query& qy = queries[y];
+            if (qx.scene != qy.scene) {
+                if (qx.scene->n_prims != qy.scene->n_prims) return qx.scene->n_prims > qy.scene->n_prims;
+                return std::less<const omg_scene*>()(qx.scene, qy.scene);
+            }
+            if (qx.params.max_bounces != qy.params.max_bounces)
+                return qx.params.max_bounces > qy.params.max_bounces;
+            return qx.params.max_time > qy.params.max_time;
         });
         b->items_host.reserve(items);
         for (uint32_t i : order)
