@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 PLAIN_TRAFFIC_BYTES = 53.04e6
 # the same for the k_trace_batch_regen_il launch of the mixed 256-query
 # workload in the shape (1,12,6,2,4) (profiles/k_trace_batch_regen_il_mixed_256q.md)
-BATCH_TRAFFIC_BYTES = 18.43e9
+BATCH_TRAFFIC_BYTES = 29.60e9
 # the town: the routed plain launch (k_trace_batch_regen_il, one-query
 # batch alike) and the single-job kernel (profiles/k_trace_town_2m.md)
 STRESS_TRAFFIC_BYTES = 156.5e9
@@ -48,8 +48,8 @@ STRESS_SINGLE_TRAFFIC_BYTES = 32.6e9
 # SQ_ACTIVE_INST_VALU / 64), from the same profiles
 PLAIN_VALU_INSTS = 6.532e8
 PLAIN_VALU_THREAD_UTIL = 0.463
-BATCH_VALU_INSTS = 6.610e10
-BATCH_VALU_THREAD_UTIL = 0.377
+BATCH_VALU_INSTS = 6.554e10
+BATCH_VALU_THREAD_UTIL = 0.381
 STRESS_VALU_INSTS = 5.345e11
 STRESS_VALU_THREAD_UTIL = 0.388
 STRESS_SINGLE_VALU_INSTS = 1.042e12
